@@ -1,0 +1,166 @@
+"""Benchmark: the n=8000 FP64 two-stage HT reduction on one MI355X.
+
+A step is one full reduction (stage 1 ab2rht_simple + stage 2) of one
+synthetic FP64 pencil (A ~ N(0,1), B = R of a QR of an N(0,1) matrix,
+Q = Z = I), generated on the GPU and kept resident in HBM; each step restores
+A, B, Q, Z from resident copies and reduces them in place.
+
+Prints one JSON line: seconds per reduction (value), GFLOP/s and % of the FP64
+peak by the 14 n^3 convention, per-stage times, backward errors of the last
+step, a roofline entry for the dominant throughput kernel measured with HIP
+events over the timed steps, and the CPU baseline (the C oracle's
+restatement of ParaHT, timed on a bounded sample on the host cores).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "n=8000 FP64 HT-reduce s & %FP64 peak; batched pencils/s; fwd+bwd/fwd ratio"
+FP64_PEAK = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
+HBM_PEAK = 8.0e12    # bytes/s
+
+
+def cpu_baseline(args):
+    """The oracle's restatement of ParaHT (stage 1 + the Householder stage 2
+    the reference driver calls), on all host cores, on a bounded sample; the
+    time is scaled by (n/n_sample)^3 to the benchmark size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from paraht_amd.pencils import random_pencil
+    ns = args.cpu_n
+    A0, B0 = random_pencil(ns)
+    t0 = time.perf_counter()
+    O.reduce(A0, B0, args.r, args.p, args.nq, stage2="householder")
+    t = time.perf_counter() - t0
+    scaled = t * (args.n / ns) ** 3
+    return {"value": scaled, "unit": "s", "cores": O.num_threads(),
+            "kind": "port",
+            "sample": f"oracle ab2rht_simple + rht2ht_householder at n={ns}, "
+                      f"r={args.r}, p={args.p}, nq={args.nq}: {t:.3f} s measured, "
+                      f"scaled by (n/{ns})^3 to n={args.n}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=8000)
+    ap.add_argument("--r", type=int, default=64)
+    ap.add_argument("--p", type=int, default=2)
+    ap.add_argument("--nq", type=int, default=16)
+    ap.add_argument("--cpu-n", type=int, default=1536)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the HT reduction of one pencil is "
+              "not sharded across GPUs in this build (--gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import paraht_amd as ph
+    n, r, p, nq = args.n, args.r, args.p, args.nq
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(1302)
+    A0 = ph.column_major(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    R = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g),
+                        mode="r")[1]
+    B0 = ph.column_major(torch.triu(R))
+    del R
+    A, B = torch.empty_like(A0), torch.empty_like(B0)
+    Q, Z = torch.empty_like(A0), torch.empty_like(A0)
+    eye = ph.column_major(torch.eye(n, dtype=torch.float64, device=dev))
+    ctx = ph.Context(0)
+
+    def step():
+        A.copy_(A0)
+        B.copy_(B0)
+        Q.copy_(eye)
+        Z.copy_(eye)
+        torch.cuda.synchronize()
+        ph.reduce(A, B, r, p, nq, Q, Z, ctx=ctx)
+
+    for i in range(args.warmup):
+        step()
+        print(f"warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    if not args.no_profile:
+        ctx.profile(True)
+    torch.cuda.synchronize()
+    t1s, t2s = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        lt = ctx.last_times()
+        t1s.append(lt["stage1_ms"] / 1e3)
+        t2s.append(lt["stage2_ms"] / 1e3)
+        print(f"step {i + 1}/{args.steps}: stage1 {t1s[-1]:.3f} s, stage2 {t2s[-1]:.3f} s",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kstats = ctx.kernel_stats() if not args.no_profile else {}
+    ctx.profile(False)
+
+    # correctness of the last step (FP64 GEMMs on the GPU)
+    ea = (torch.linalg.norm(Q.T @ A0 @ Z - A) / torch.linalg.norm(A0)).item()
+    eb = (torch.linalg.norm(Q.T @ B0 @ Z - B) / torch.linalg.norm(B0)).item()
+    zH = int(torch.count_nonzero(torch.tril(A, -2)).item())
+    zT = int(torch.count_nonzero(torch.tril(B, -1)).item())
+
+    ms = wall / args.steps * 1e3
+    t_red = (sum(t1s) + sum(t2s)) / args.steps
+    flops = 14.0 * n ** 3
+    t1 = sum(t1s) / args.steps
+    t2 = sum(t2s) / args.steps
+    out = {
+        "metric": METRIC, "value": t_red, "unit": "s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"one FP64 pencil n={n}, r={r}, p={p}, nq={nq}; "
+                               "A~N(0,1), B=R of QR of N(0,1), Q=Z=I",
+                   "n": n, "r": r, "p": p, "nq": nq, "stage2": "givens"},
+        "gflops": flops / t_red / 1e9,
+        "pct_fp64_peak": 100.0 * flops / t_red / FP64_PEAK,
+        "stage1_s": t1, "stage2_s": t2,
+        "stage1_pct_fp64_peak": 100.0 * (28.0 / 3.0) * n ** 3 / t1 / FP64_PEAK,
+        "stage2_hbm_gbs": 32.0 * n ** 3 / nq / t2 / 1e9,
+        "backward_errors": [ea, eb], "nonzeros_below_structure": [zH, zT],
+    }
+    if kstats:
+        tot = sum(v[0] for v in kstats.values()) or 1.0
+        out["kernels"] = {k: {"ms": v[0] / args.steps, "launches": v[1] // args.steps,
+                              "share": v[0] / tot} for k, v in kstats.items() if v[1]}
+        dom = max(kstats, key=lambda k: kstats[k][0])
+        out["dominant_kernel"] = dom
+        # roofline of the dominant throughput kernel (the generation kernels
+        # are latency-bound chains and have no throughput roofline)
+        cands = {k: v for k, v in kstats.items() if k in ("wy_apply", "s2_rows") and v[1]}
+        if cands:
+            k = max(cands, key=lambda x: cands[x][0])
+            ms_k, cnt, fl, by = cands[k]
+            if k == "wy_apply":
+                ach = fl / cnt / (ms_k / cnt / 1e3) / 1e12
+                out["roofline"] = {"kernel": "k_apply_blocks (FP64 MFMA compact-WY)",
+                                   "bound": "mfma", "achieved": ach, "peak": 78.6,
+                                   "unit": "TFLOP/s", "frac": ach / 78.6, "traffic": None}
+            else:
+                ach = by / cnt / (ms_k / cnt / 1e3) / 1e9
+                out["roofline"] = {"kernel": "k_s2_rows (rotation sweeps of Q, Z, A, B)",
+                                   "bound": "hbm", "achieved": ach, "peak": 8000.0,
+                                   "unit": "GB/s", "frac": ach / 8000.0, "traffic": None}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

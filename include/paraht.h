@@ -1,0 +1,122 @@
+/*
+ * paraht.h -- C-ABI of libparaht_hip.so, the MI355X (gfx950) implementation of
+ * ParaHT's two-stage Hessenberg-triangular (HT) reduction of a pencil (A,B).
+ *
+ * All matrices are FP64, column-major, with leading dimensions, and are
+ * updated in place exactly as the reference's Fortran routines update theirs:
+ * on return Q_out^T * A_in * Z_out == A_out (with Q_in, Z_in accumulated), and
+ * the same for B.  Arguments are passed by value (the Fortran drop-in modules
+ * pass them with iso_c_binding `value`).
+ *
+ * Return value: 0 on success; -i if the i-th argument had an illegal value
+ * (LAPACK convention, counting from 1 in the order of the reference's Fortran
+ * argument list); a positive PARAHT_ERR_* code on a device or allocation
+ * failure.  Calls block the host until the device work has finished.
+ */
+#ifndef PARAHT_H
+#define PARAHT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (positive = failure outside the arguments) ---- */
+#define PARAHT_OK 0
+#define PARAHT_ERR_HIP 1      /* a HIP runtime call failed */
+#define PARAHT_ERR_NOMEM 2    /* device or host allocation failed */
+#define PARAHT_ERR_NODEV 3    /* no usable gfx950 device */
+#define PARAHT_ERR_UNSUPPORTED 4 /* parameters outside what the kernels support */
+
+/* ---- flags ---- */
+#define PARAHT_TIME_COMPONENTS 0x1 /* print per-phase times like the reference's
+                                       time_components argument */
+#define PARAHT_STAGE2_GIVENS 0x2   /* stage 2 with Givens rotations
+                                       (rht2ht_givens); default is the
+                                       Householder variant's entry point */
+
+/* Opaque per-device context: stream, workspaces and phase timers.
+ * A context is not re-entrant; use one per host thread. */
+typedef struct paraht_ctx paraht_ctx;
+
+int paraht_ctx_create(paraht_ctx** ctx, int device);
+int paraht_ctx_destroy(paraht_ctx* ctx);
+/* Use an existing hipStream_t (as void*) for all work of this context;
+ * NULL restores the context's own stream. */
+int paraht_ctx_set_stream(paraht_ctx* ctx, void* stream);
+/* Phase times (milliseconds) of the last call on this context:
+ * [0] stage 1 total, [1] stage-1 left generation, [2] stage-1 right
+ * generation chain, [3] stage 2 total, [4] stage-2 generation.
+ * Returns the number of entries written. */
+int paraht_ctx_last_times(paraht_ctx* ctx, double* ms, int nmax);
+
+/* ---- Per-kernel timing ----
+ * The device counterpart of the reference's per-task timers (time_tasks,
+ * src/ab2rht.f90:26, src/rht2ht.f90:27,1366): with profiling enabled every
+ * kernel launch of the context is bracketed by HIP events on the context's
+ * stream and summed per kernel class, together with the launch's algorithmic
+ * flops (compact-WY appliers: 4*len*k*width per block) and bytes (stage-2
+ * row sweeps: every touched element read and written once). */
+#define PARAHT_K_WY_APPLY 0    /* FP64-MFMA compact-WY block applier */
+#define PARAHT_K_S1_PANEL_QR 1 /* stage-1 left generation (panel QRs) */
+#define PARAHT_K_S1_RQ_GEN 2   /* stage-1 right generation (block RQ + LQ) */
+#define PARAHT_K_S2_GEN 3      /* stage-2 rotation generation (band window) */
+#define PARAHT_K_S2_ROWS 4     /* stage-2 row sweeps (A,B above window; Q, Z) */
+#define PARAHT_K_S2_COLS 5     /* stage-2 column sweeps (A,B right of window) */
+#define PARAHT_K_OTHER 6       /* small helpers (explicit zeroing) */
+#define PARAHT_NKERNELS 7
+/* Enable (1) or disable (0) profiling; both reset the accumulated sums. */
+int paraht_ctx_profile(paraht_ctx* ctx, int enable);
+/* Accumulated time, launch count and algorithmic flops/bytes of a kernel
+ * class since profiling was enabled (waits for the context's stream). */
+int paraht_ctx_kernel_stats(paraht_ctx* ctx, int kernel, double* total_ms,
+                            long long* launches, double* flops, double* bytes);
+
+/* ---- Stage 1: (A,B) -> r-Hessenberg-triangular ----
+ * Replaces ab2rht_simple(n, A, lda, B, ldb, Q, ldq, Z, ldz, nb, p,
+ * time_components), src/ab2rht.f90:8-13.  nb is the bandwidth r, p the number
+ * of nb-row blocks per reflector block (p >= 2).
+ * Host pointers: the arrays are copied to the device and back. */
+int paraht_ab2rht(int n, double* A, int lda, double* B, int ldb, double* Q,
+                  int ldq, double* Z, int ldz, int nb, int p, int flags);
+
+/* ---- Stage 2: r-Hessenberg-triangular -> Hessenberg-triangular ----
+ * Replaces rht2ht_householder(n, A, lda, B, ldb, Q, ldq, Z, ldz, r, nq,
+ * time_components), src/rht2ht.f90:1346-1351, and with PARAHT_STAGE2_GIVENS
+ * rht2ht_givens(n, A, lda, B, ldb, Q, ldq, Z, ldz, r, nq),
+ * src/rht2ht.f90:696-700.  A must have lower bandwidth <= r, B upper
+ * triangular.  Host pointers. */
+int paraht_rht2ht(int n, double* A, int lda, double* B, int ldb, double* Q,
+                  int ldq, double* Z, int ldz, int r, int nq, int flags);
+
+/* Replaces rht2ht_householder_unblocked(n, A, lda, B, ldb, Q, ldq, Z, ldz, r),
+ * src/rht2ht.f90:2053-2057 (the unblocked specification of stage 2). */
+int paraht_rht2ht_unblocked(int n, double* A, int lda, double* B, int ldb,
+                            double* Q, int ldq, double* Z, int ldz, int r);
+
+/* Both stages with one host<->device round trip (the reference driver's
+ * ab2rht_simple + rht2ht_* sequence, src/profilereduction.f90:136,144). */
+int paraht_reduce(int n, double* A, int lda, double* B, int ldb, double* Q,
+                  int ldq, double* Z, int ldz, int r, int p, int nq, int flags);
+
+/* ---- Device-pointer variants (arrays already resident in HBM) ----
+ * Same semantics as the host versions; pointers are device addresses.  The
+ * work is enqueued on the context's stream and the call waits for it. */
+int paraht_ab2rht_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
+                      int ldb, double* dQ, int ldq, double* dZ, int ldz,
+                      int nb, int p, int flags);
+int paraht_rht2ht_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
+                      int ldb, double* dQ, int ldq, double* dZ, int ldz, int r,
+                      int nq, int flags);
+int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
+                      int ldb, double* dQ, int ldq, double* dZ, int ldz, int r,
+                      int p, int nq, int flags);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int paraht_version(void);
+/* Human-readable text of a status code. */
+const char* paraht_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARAHT_H */

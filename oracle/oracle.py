@@ -1,0 +1,230 @@
+"""CPU oracle for the two-stage HT reduction -- TEST INFRASTRUCTURE ONLY.
+
+Only the test-suite, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this module, and only as the checker.  The product path
+(``paraht_amd`` over ``libparaht_hip.so``) never imports it.
+
+Contents
+--------
+* ctypes bindings of ``liboracle.so``, the C restatement of the reference:
+  ``ab2rht_simple`` (src/ab2rht.f90:8-495), ``rht2ht_householder``
+  (src/rht2ht.f90:1346-2051), ``rht2ht_givens`` (src/rht2ht.f90:696-1344) and
+  ``rht2ht_householder_unblocked`` (src/rht2ht.f90:2053-2180), with the
+  reference's argument lists and in-place semantics on column-major arrays;
+* LAPACK DGGHD3 from the host's LAPACK (scipy's OpenBLAS) -- an independent
+  library routine that produces the same Hessenberg-triangular form up to
+  diagonal signs (generalized implicit-Q theorem), used to pin the oracle;
+* the reference's result metrics (src/profilereduction.f90:164-174) plus the
+  checks the reference lacks: orthogonality, exact structural zeros, sign
+  normalisation and generalized eigenvalues in the chordal metric.
+"""
+from __future__ import annotations
+
+import ctypes
+import glob
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ci = ctypes.c_int
+        sig = [ci, dp, ci, dp, ci, dp, ci, dp, ci, ci, ci]
+        for name in ("oracle_ab2rht_simple", "oracle_rht2ht_householder",
+                     "oracle_rht2ht_givens"):
+            f = getattr(L, name)
+            f.argtypes = sig
+            f.restype = ci
+        L.oracle_rht2ht_householder_unblocked.argtypes = sig[:-1]
+        L.oracle_rht2ht_householder_unblocked.restype = ci
+        L.oracle_num_threads.restype = ci
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.f_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _call(name, n, A, B, Q, Z, *ints):
+    info = getattr(lib(), name)(n, _ptr(A), A.shape[0], _ptr(B), B.shape[0],
+                                _ptr(Q), Q.shape[0], _ptr(Z), Z.shape[0], *ints)
+    if info != 0:
+        raise ValueError(f"{name}: illegal argument {-info}")
+
+
+def ab2rht_simple(n, A, B, Q, Z, nb, p):
+    """In place, like src/ab2rht.f90:8 (arrays are Fortran-ordered float64)."""
+    _call("oracle_ab2rht_simple", n, A, B, Q, Z, nb, p)
+
+
+def rht2ht_householder(n, A, B, Q, Z, r, nq):
+    """In place, like src/rht2ht.f90:1346."""
+    _call("oracle_rht2ht_householder", n, A, B, Q, Z, r, nq)
+
+
+def rht2ht_givens(n, A, B, Q, Z, r, nq):
+    """In place, like src/rht2ht.f90:696."""
+    _call("oracle_rht2ht_givens", n, A, B, Q, Z, r, nq)
+
+
+def rht2ht_householder_unblocked(n, A, B, Q, Z, r):
+    """In place, like src/rht2ht.f90:2053."""
+    _call("oracle_rht2ht_householder_unblocked", n, A, B, Q, Z, r)
+
+
+def num_threads() -> int:
+    return int(lib().oracle_num_threads())
+
+
+def reduce(A0, B0, r, p, nq, stage2="givens"):
+    """Both stages on copies: returns H, T, Q, Z with Q^T A0 Z = H."""
+    n = A0.shape[0]
+    A = np.asfortranarray(A0, dtype=np.float64).copy(order="F")
+    B = np.asfortranarray(B0, dtype=np.float64).copy(order="F")
+    Q = np.eye(n, order="F")
+    Z = np.eye(n, order="F")
+    ab2rht_simple(n, A, B, Q, Z, r, p)
+    if stage2 == "givens":
+        rht2ht_givens(n, A, B, Q, Z, r, nq)
+    elif stage2 == "householder":
+        rht2ht_householder(n, A, B, Q, Z, r, nq)
+    elif stage2 == "unblocked":
+        rht2ht_householder_unblocked(n, A, B, Q, Z, r)
+    else:
+        raise ValueError(stage2)
+    return A, B, Q, Z
+
+
+# --------------------------------------------------------------------------
+# Host LAPACK DGGHD3 (an independent library routine used as a pin)
+# --------------------------------------------------------------------------
+_LAPACK = None
+
+
+def _lapack():
+    global _LAPACK
+    if _LAPACK is None:
+        import scipy
+        d = os.path.join(os.path.dirname(scipy.__file__), os.pardir, "scipy.libs")
+        cands = sorted(glob.glob(os.path.join(d, "libscipy_openblas-*.so")))
+        if not cands:
+            raise RuntimeError("scipy's OpenBLAS not found")
+        _LAPACK = ctypes.CDLL(cands[0])
+    return _LAPACK
+
+
+def dgghd3(A0, B0):
+    """LAPACK DGGHD3('I','I',n,1,n,...) on copies; returns H, T, Q, Z."""
+    L = _lapack()
+    f = L.scipy_dgghd3_
+    n = A0.shape[0]
+    A = np.array(A0, dtype=np.float64, order="F")
+    B = np.array(B0, dtype=np.float64, order="F")
+    Q = np.zeros((n, n), order="F")
+    Z = np.zeros((n, n), order="F")
+    ci = ctypes.c_int
+    nn, ilo, ihi, ld = ci(n), ci(1), ci(n), ci(max(1, n))
+    info = ci(0)
+    lwork = ci(-1)
+    wq = np.zeros(1)
+    cq = ctypes.c_char(b"I")
+    args_tail = (ctypes.c_size_t(1), ctypes.c_size_t(1))
+    f(ctypes.byref(cq), ctypes.byref(cq), ctypes.byref(nn), ctypes.byref(ilo),
+      ctypes.byref(ihi), _ptr(A), ctypes.byref(ld), _ptr(B), ctypes.byref(ld),
+      _ptr(Q), ctypes.byref(ld), _ptr(Z), ctypes.byref(ld),
+      wq.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(lwork),
+      ctypes.byref(info), *args_tail)
+    lw = max(1, int(wq[0]))
+    work = np.zeros(lw)
+    lwork = ci(lw)
+    f(ctypes.byref(cq), ctypes.byref(cq), ctypes.byref(nn), ctypes.byref(ilo),
+      ctypes.byref(ihi), _ptr(A), ctypes.byref(ld), _ptr(B), ctypes.byref(ld),
+      _ptr(Q), ctypes.byref(ld), _ptr(Z), ctypes.byref(ld),
+      work.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+      ctypes.byref(lwork), ctypes.byref(info), *args_tail)
+    if info.value != 0:
+        raise RuntimeError(f"dgghd3 info={info.value}")
+    return A, B, Q, Z
+
+
+# --------------------------------------------------------------------------
+# Metrics
+# --------------------------------------------------------------------------
+def backward_errors(A0, B0, H, T, Q, Z):
+    """||Q^T A0 Z - H||_F/||A0||_F and the same for B (profilereduction.f90:164-174)."""
+    ea = np.linalg.norm(Q.T @ A0 @ Z - H) / np.linalg.norm(A0)
+    eb = np.linalg.norm(Q.T @ B0 @ Z - T) / np.linalg.norm(B0)
+    return float(ea), float(eb)
+
+
+def orthogonality(Q):
+    n = Q.shape[0]
+    return float(np.linalg.norm(Q.T @ Q - np.eye(n)))
+
+
+def below_band(M, band):
+    """max |M(i,j)| over i > j + band (band=1: Hessenberg, 0: triangular)."""
+    n = M.shape[0]
+    mask = np.tril(np.ones((n, n), dtype=bool), -band - 1)
+    return float(np.max(np.abs(M[mask]))) if mask.any() else 0.0
+
+
+def sign_normalize(H, T, Q, Z, band=1):
+    """Fix the diagonal-sign freedom of an (r-)Hessenberg-triangular form with
+    Q(:,1:band) fixed: choose D, E = diag(+-1) so that T(j,j) >= 0 and
+    H(j+band, j) >= 0; returns D H E, D T E, Q D, Z E."""
+    n = H.shape[0]
+    d = np.ones(n)
+    e = np.ones(n)
+    for j in range(n):
+        t = T[j, j]
+        if t == 0.0:
+            t = H[j, j] if H[j, j] != 0.0 else 1.0
+        e[j] = d[j] * (1.0 if t > 0 else -1.0)
+        if j + band < n:
+            h = H[j + band, j]
+            d[j + band] = e[j] * (1.0 if h >= 0 else -1.0)
+    return (d[:, None] * H * e[None, :], d[:, None] * T * e[None, :],
+            Q * d[None, :], Z * e[None, :])
+
+
+def chordal_eig_distance(A0, B0, H, T):
+    """Largest chordal distance between matched generalized eigenvalues of
+    (A0,B0) and (H,T) (infinite eigenvalues included)."""
+    import scipy.linalg as sl
+    from scipy.optimize import linear_sum_assignment
+
+    w1 = sl.eigvals(A0, B0, homogeneous_eigvals=True)
+    w2 = sl.eigvals(H, T, homogeneous_eigvals=True)
+
+    def unit(w):
+        a, b = w
+        nrm = np.sqrt(np.abs(a) ** 2 + np.abs(b) ** 2)
+        nrm[nrm == 0] = 1.0
+        return a / nrm, b / nrm
+
+    a1, b1 = unit(w1)
+    a2, b2 = unit(w2)
+    # chordal distance |a1 b2 - b1 a2| for unit-normalised pairs
+    D = np.abs(a1[:, None] * b2[None, :] - b1[:, None] * a2[None, :])
+    ri, ci = linear_sum_assignment(D)
+    return float(D[ri, ci].max()) if len(ri) else 0.0

@@ -1,0 +1,165 @@
+// Compact-WY block-reflector applier on FP64 MFMA (v_mfma_f64_16x16x4_f64).
+//
+// Replaces the reference's DLARFB call sites of both stages:
+//   left  C <- H^T C : ab2rht.f90:161,202; rht2ht.f90:1680,1748,1759,1862,1940,1945
+//   right C <- C H   : ab2rht.f90:241,327,335,359,362,412,415,455,458;
+//                      rht2ht.f90:1841,1917,1982,2015
+// with H = I - V T V^T given explicitly (V: len x k, T: k x k).
+//
+// Work split: a workgroup of 4 waves owns 64 consecutive indices of the
+// "other" dimension (columns for a left update, rows for a right update);
+// each wave owns 16 of them and is independent of the other waves.  For every
+// block (applied in list order, which is the order the reference applies
+// them) a wave computes
+//   W  = V^T C_slice          (k x 16, reduction over the block span)
+//   W2 = T^T W
+//   C_slice -= V W2
+// with 16x16 accumulator tiles.  Element i of a lane's accumulator holds row
+// 4*i + lane/16 of its tile, so the i-th MFMA of a 16-deep chunk takes its
+// reduction indices 4*i + slot from register i directly: the reduction index
+// is permuted within each chunk and no data moves across lanes.
+#include "device_util.hpp"
+#include "kernels.hpp"
+
+namespace pht {
+
+template <int SIDE>
+__device__ __forceinline__ size_t cidx(int span, int other, int ldc) {
+  return SIDE == kLeftT ? PHT_IDX(span, other, ldc) : PHT_IDX(other, span, ldc);
+}
+
+template <int SIDE, int NKT>
+__global__ __launch_bounds__(256) void k_apply_blocks(
+    double* __restrict__ C, int ldc, const double* __restrict__ V, int ldv,
+    const double* __restrict__ T, int ldt, const ApplyBlock* __restrict__ blocks,
+    int nblocks, int other_begin) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int l16 = lane & 15;
+  const int q = lane >> 4;
+  const int o0 = other_begin + (blockIdx.x * 4 + wave) * 16;
+  const int oi = o0 + l16;
+
+  for (int b = 0; b < nblocks; ++b) {
+    const ApplyBlock bk = blocks[b];
+    const bool active = (o0 < bk.hi) && (o0 + 16 > bk.lo);
+    if (active) {
+      const bool omask = (oi >= bk.lo) && (oi < bk.hi);
+      const double* Vb = V + bk.voff;
+      const double* Tb = T + bk.toff;
+      const int len = bk.len, k = bk.k, st = bk.start;
+      const int nct = (len + 15) >> 4;
+
+      dbl4 W[NKT];
+#pragma unroll
+      for (int t = 0; t < NKT; ++t) W[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+      // phase 1: W = V^T C_slice
+      for (int ct = 0; ct < nct; ++ct) {
+        double c[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int s = 16 * ct + 4 * i + q;
+          c[i] = (omask && s < len) ? C[cidx<SIDE>(st + s, oi, ldc)] : 0.0;
+        }
+#pragma unroll
+        for (int kt = 0; kt < NKT; ++kt) {
+          const int m = 16 * kt + l16;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int s = 16 * ct + 4 * i + q;
+            const double a = (s < len && m < k) ? Vb[PHT_IDX(s, m, ldv)] : 0.0;
+            W[kt] = mfma16(a, c[i], W[kt]);
+          }
+        }
+      }
+      // phase 2: W2 = T^T W
+      dbl4 W2[NKT];
+#pragma unroll
+      for (int mt = 0; mt < NKT; ++mt) {
+        W2[mt] = dbl4{0.0, 0.0, 0.0, 0.0};
+        const int m = 16 * mt + l16;
+#pragma unroll
+        for (int kt = 0; kt < NKT; ++kt) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int kk = 16 * kt + 4 * i + q;
+            const double a = (kk < k && m < k) ? Tb[PHT_IDX(kk, m, ldt)] : 0.0;
+            W2[mt] = mfma16(a, W[kt][i], W2[mt]);
+          }
+        }
+      }
+      // phase 3: C_slice -= V W2
+      for (int ct = 0; ct < nct; ++ct) {
+        dbl4 acc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int s = 16 * ct + 4 * i + q;
+          acc[i] = (omask && s < len) ? C[cidx<SIDE>(st + s, oi, ldc)] : 0.0;
+        }
+        const int srow = 16 * ct + l16;
+#pragma unroll
+        for (int kt = 0; kt < NKT; ++kt) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int m = 16 * kt + 4 * i + q;
+            const double a =
+                (srow < len && m < k) ? -Vb[PHT_IDX(srow, m, ldv)] : 0.0;
+            acc = mfma16(a, W2[kt][i], acc);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int s = 16 * ct + 4 * i + q;
+          if (omask && s < len) C[cidx<SIDE>(st + s, oi, ldc)] = acc[i];
+        }
+      }
+    }
+    // consecutive blocks overlap in the span dimension: make this wave's
+    // stores visible before the next block's loads
+    __syncthreads();
+  }
+}
+
+template <int SIDE>
+static hipError_t launch_side(hipStream_t st, double* C, int ldc,
+                              const double* V, int ldv, const double* T,
+                              int ldt, const ApplyBlock* dblocks, int nblocks,
+                              int kmax, int other_begin, int other_end) {
+  const int extent = other_end - other_begin;
+  if (extent <= 0 || nblocks <= 0) return hipSuccess;
+  dim3 grid((extent + 63) / 64), block(256);
+  const int nkt = (kmax + 15) / 16;
+  switch (nkt) {
+#define PHT_CASE(N)                                                         \
+  case N:                                                                   \
+    hipLaunchKernelGGL((k_apply_blocks<SIDE, N>), grid, block, 0, st, C, ldc, \
+                       V, ldv, T, ldt, dblocks, nblocks, other_begin);       \
+    break;
+    PHT_CASE(1)
+    PHT_CASE(2)
+    PHT_CASE(3)
+    PHT_CASE(4)
+    PHT_CASE(5)
+    PHT_CASE(6)
+    PHT_CASE(7)
+    PHT_CASE(8)
+#undef PHT_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t apply_blocks(hipStream_t st, Side side, double* C, int ldc,
+                        const double* V, int ldv, const double* T, int ldt,
+                        const ApplyBlock* dblocks, int nblocks, int kmax,
+                        int other_begin, int other_end) {
+  if (side == kLeftT)
+    return launch_side<kLeftT>(st, C, ldc, V, ldv, T, ldt, dblocks, nblocks,
+                               kmax, other_begin, other_end);
+  return launch_side<kRightN>(st, C, ldc, V, ldv, T, ldt, dblocks, nblocks,
+                              kmax, other_begin, other_end);
+}
+
+}  // namespace pht

@@ -1,0 +1,139 @@
+// Device-side helpers shared by the gfx950 kernels of libparaht_hip.so.
+// All matrices are FP64 column-major; indices inside kernels are 0-based.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace pht {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+#define PHT_IDX(i, j, ld) ((size_t)(j) * (size_t)(ld) + (size_t)(i))
+
+// One v_mfma_f64_16x16x4_f64: D(16x16) += A(16x4) * B(4x16).
+// Lane l supplies A[l%16][l/16] and B[l/16][l%16]; D[4*i + l/16][l%16] is
+// element i of the lane's dbl4 (checked on gfx950).
+__device__ __forceinline__ dbl4 mfma16(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum over the whole workgroup (blockDim.x a multiple of 64, <= 1024).
+// red must hold >= blockDim.x/64 doubles of LDS.  Contains barriers.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nw; ++w) s += red[w];
+  return s;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  v = wave_max(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nw; ++w) s = fmax(s, red[w]);
+  return s;
+}
+
+// Householder reflector from alpha and ||x|| (DLARFG semantics:
+// beta = -sign(alpha)*||[alpha;x]||, tau = (beta-alpha)/beta,
+// v = x * scal with scal = 1/(alpha-beta); tau = 0 and scal = 1 when x = 0).
+// Tiny |beta| is handled by rescaling with 1/safmin as DLARFG specifies.
+struct Refl {
+  double tau, scal, beta;
+};
+
+__device__ __forceinline__ double dlapy2_d(double x, double y) {
+  double xa = fabs(x), ya = fabs(y);
+  double w = fmax(xa, ya), z = fmin(xa, ya);
+  if (z == 0.0 || w > DBL_MAX) return w;
+  double q = z / w;
+  return w * sqrt(1.0 + q * q);
+}
+
+__device__ __forceinline__ Refl make_refl(double alpha, double xnorm) {
+  Refl R;
+  if (xnorm == 0.0) {
+    R.tau = 0.0;
+    R.scal = 1.0;
+    R.beta = alpha;
+    return R;
+  }
+  double beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
+  const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+  double a = alpha, xn = xnorm, mult = 1.0;
+  int knt = 0;
+  if (fabs(beta) < safmin) {
+    const double rsafmn = 1.0 / safmin;
+    do {
+      ++knt;
+      xn *= rsafmn;
+      beta *= rsafmn;
+      a *= rsafmn;
+      mult *= rsafmn;
+    } while (fabs(beta) < safmin && knt < 20);
+    beta = -copysign(dlapy2_d(a, xn), a);
+  }
+  R.tau = (beta - a) / beta;
+  R.scal = mult / (a - beta);  // applied to the unscaled x
+  for (int j = 0; j < knt; ++j) beta *= safmin;
+  R.beta = beta;
+  return R;
+}
+
+// Plane rotation [c s; -s c][f; g] = [r; 0] with c >= 0 and r carrying the
+// sign of f (Anderson, "Algorithm 978", ACM TOMS 2017).
+__device__ __forceinline__ void dlartg_d(double f, double g, double& c,
+                                         double& s, double& r) {
+  const double safmin = DBL_MIN;
+  const double safmax = 1.0 / DBL_MIN;
+  const double rtmin = 1.4916681462400413e-154;  // sqrt(DBL_MIN)
+  const double rtmax = 9.4807519081091774e+153;  // sqrt(safmax/2)
+  double f1 = fabs(f), g1 = fabs(g);
+  if (g == 0.0) {
+    c = 1.0;
+    s = 0.0;
+    r = f;
+  } else if (f == 0.0) {
+    c = 0.0;
+    s = copysign(1.0, g);
+    r = g1;
+  } else if (f1 > rtmin && f1 < rtmax && g1 > rtmin && g1 < rtmax) {
+    double d = sqrt(f * f + g * g);
+    c = f1 / d;
+    r = copysign(d, f);
+    s = g / r;
+  } else {
+    double u = fmin(safmax, fmax(safmin, fmax(f1, g1)));
+    double fs = f / u, gs = g / u;
+    double d = sqrt(fs * fs + gs * gs);
+    c = fabs(fs) / d;
+    double rr = copysign(d, f);
+    s = gs / rr;
+    r = rr * u;
+  }
+}
+
+}  // namespace pht

@@ -1,0 +1,69 @@
+// Host-side declarations of the kernel launchers of libparaht_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pht {
+
+// One compact-WY block reflector H = I - V*T*V^T to apply to a matrix C.
+// The block acts on the index span [start, start+len) of the transformed
+// dimension (rows for a left update, columns for a right update) and is
+// applied on the range [lo, hi) of the other dimension.  V is explicit
+// (len x k, unit entries and zeros stored), T is k x k with zeros stored
+// outside its triangle.  All indices are 0-based.
+struct ApplyBlock {
+  int start, len, k, lo, hi, pad_;
+  long long voff, toff;
+};
+
+enum Side { kLeftT = 0, kRightN = 1 };
+
+// Applies blocks[0..nblocks) in order: C <- H^T C (kLeftT) or C <- C H
+// (kRightN), on FP64 MFMA tiles.  [other_begin, other_end) bounds the other
+// dimension over all blocks (it sets the grid).  kmax = max k over blocks.
+hipError_t apply_blocks(hipStream_t st, Side side, double* C, int ldc,
+                        const double* V, int ldv, const double* T, int ldt,
+                        const ApplyBlock* dblocks, int nblocks, int kmax,
+                        int other_begin, int other_end);
+
+// ---- stage 1 (ab2rht_simple) ----
+// Bottom-up block QRs of the panel A(jb+nb:n, jb:jb+nb-1) (0-based jb);
+// writes explicit V (ldv = Pmax rows per block, vstride per block) and T.
+hipError_t s1_panel_qr(hipStream_t st, double* A, int lda, int n, int jb,
+                       int nb, int p, int nblocks, double* Vt, int ldv,
+                       long long vstride, double* Tt, int ldt,
+                       long long tstride);
+// RQ-based right generator for one B block B(i0:i0+len, i0:i0+len); writes
+// explicit column-form V (len x k) and T (k x k); k = nb if len > nb else len.
+// scratch: device buffer of >= len*len + nb*len doubles (used when the block
+// does not fit in LDS).
+hipError_t s1_rq_gen(hipStream_t st, const double* B, int ldb, int i0,
+                     int len, int nb, double* V, int ldv, double* T, int ldt,
+                     double* scratch);
+// Sets B(j+1:i0+len-1, j) = 0 for j in [i0, i0+ncols).
+hipError_t zero_below(hipStream_t st, double* B, int ldb, int i0, int len,
+                      int ncols);
+
+// ---- stage 2 (rht2ht_givens semantics) ----
+struct S2Params {
+  int n, r, nq, jblock;  // jblock 1-based like the reference
+  int nblocks;           // bulge blocks of this sweep
+  int ldr;               // r + nq
+  double* M;             // nblocks x ldr x ldr workspace (deferred rotations)
+};
+hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
+                       const S2Params& P, double* CU, double* SU, double* CV,
+                       double* SV);
+// Remaining right rotations (rows of A,B above the generation window, all
+// rows of Z) and left rotations seen from the right (all rows of Q).
+hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
+                         int ldb, double* Q, int ldq, double* Z, int ldz,
+                         const S2Params& P, const double* CU, const double* SU,
+                         const double* CV, const double* SV);
+// Remaining left rotations (columns of A,B right of the generation window).
+hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
+                         int ldb, const S2Params& P, const double* CU,
+                         const double* SU);
+
+}  // namespace pht

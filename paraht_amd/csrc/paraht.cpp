@@ -1,0 +1,621 @@
+// Host orchestration and C-ABI of libparaht_hip.so (see include/paraht.h).
+//
+// Stage 1 follows ab2rht_simple (src/ab2rht.f90:8-495) panel by panel:
+// left generation -> left updates of B, A and Q -> the bottom-up chain of
+// right generators, each immediately applied to rows 1..ib2 of B (the
+// reference's lookahead + rest of B) -> right updates of A and Z.
+// Stage 2 follows rht2ht_givens (src/rht2ht.f90:696-1344) sweep by sweep:
+// generation in the band window -> remaining row updates (A, B above the
+// window; Q, Z) -> remaining column updates (A, B right of the window).
+// All work of a call is enqueued on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/paraht.h"
+#include "kernels.hpp"
+
+using pht::ApplyBlock;
+
+// Per-kernel-class event timing (enabled by paraht_ctx_profile).
+struct KernelRec {
+  int kid;
+  int e0, e1;  // indices into the event pool
+  double flops, bytes;
+};
+
+struct paraht_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  double times[8] = {0};
+  bool profiling = false;
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+  std::vector<KernelRec> recs;
+  // accumulated per kernel id
+  double k_ms[PARAHT_NKERNELS] = {0}, k_flops[PARAHT_NKERNELS] = {0},
+         k_bytes[PARAHT_NKERNELS] = {0};
+  long long k_n[PARAHT_NKERNELS] = {0};
+};
+
+namespace {
+
+bool verbose() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("PARAHT_VERBOSE");
+    v = (e && *e && *e != '0') ? 1 : 0;
+  }
+  return v == 1;
+}
+
+int hip_fail(hipError_t e, const char* what, int line) {
+  if (verbose())
+    fprintf(stderr, "paraht: HIP error %d (%s) in %s (paraht.cpp:%d)\n", (int)e,
+            hipGetErrorString(e), what, line);
+  return e == hipErrorOutOfMemory ? PARAHT_ERR_NOMEM : PARAHT_ERR_HIP;
+}
+
+#define CK(expr)                                           \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
+    if (_e != hipSuccess) return hip_fail(_e, #expr, __LINE__); \
+  } while (0)
+
+// grow-only device workspace of the context
+int ensure_ws(paraht_ctx* c, size_t bytes) {
+  if (c->ws_bytes >= bytes) return 0;
+  if (c->ws) {
+    CK(hipStreamSynchronize(c->stream));
+    CK(hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+  }
+  CK(hipMalloc(&c->ws, bytes));
+  c->ws_bytes = bytes;
+  return 0;
+}
+
+int check_common(int n, int lda, int ldb, int ldq, int ldz) {
+  if (n < 0) return -1;
+  if (lda < std::max(1, n)) return -3;
+  if (ldb < std::max(1, n)) return -5;
+  if (ldq < std::max(1, n)) return -7;
+  if (ldz < std::max(1, n)) return -9;
+  return 0;
+}
+
+// Event pair around one launch when profiling; folds finished records into
+// the per-kernel sums whenever the pool runs low.
+int prof_flush(paraht_ctx* c) {
+  if (c->recs.empty()) return 0;
+  CK(hipStreamSynchronize(c->stream));
+  for (const KernelRec& r : c->recs) {
+    float t = 0.f;
+    CK(hipEventElapsedTime(&t, c->pool[r.e0], c->pool[r.e1]));
+    c->k_ms[r.kid] += t;
+    c->k_n[r.kid] += 1;
+    c->k_flops[r.kid] += r.flops;
+    c->k_bytes[r.kid] += r.bytes;
+  }
+  c->recs.clear();
+  c->pool_used = 0;
+  return 0;
+}
+
+int prof_event(paraht_ctx* c, int* idx) {
+  if (c->pool_used + 2 > c->pool.size()) {
+    if (c->pool.size() >= 8192) {
+      if (int e = prof_flush(c)) return e;
+    } else {
+      size_t grow = c->pool.size() ? c->pool.size() : 512;
+      for (size_t i = 0; i < grow; ++i) {
+        hipEvent_t ev;
+        CK(hipEventCreate(&ev));
+        c->pool.push_back(ev);
+      }
+    }
+  }
+  *idx = (int)c->pool_used++;
+  CK(hipEventRecord(c->pool[*idx], c->stream));
+  return 0;
+}
+
+struct Prof {
+  paraht_ctx* c;
+  int kid, e0 = -1;
+  double flops, bytes;
+  Prof(paraht_ctx* cx, int k, double f, double b) : c(cx), kid(k), flops(f), bytes(b) {}
+  int begin() { return c->profiling ? prof_event(c, &e0) : 0; }
+  int end() {
+    if (!c->profiling) return 0;
+    int e1 = -1;
+    if (int e = prof_event(c, &e1)) return e;
+    c->recs.push_back({kid, e0, e1, flops, bytes});
+    return 0;
+  }
+};
+
+// Runs `launch` between the profiling events of kernel class `kid`.
+#define PROF(kid, flops, bytes, launch)                   \
+  do {                                                    \
+    Prof _p(c, (kid), (flops), (bytes));                  \
+    if (int _e = _p.begin()) return _e;                   \
+    CK(launch);                                           \
+    if (int _e = _p.end()) return _e;                     \
+  } while (0)
+
+// algorithmic flops of a list of compact-WY blocks: 4*len*k*width each
+double wy_flops(const std::vector<ApplyBlock>& v, size_t first, int cnt) {
+  double f = 0;
+  for (int i = 0; i < cnt; ++i) {
+    const ApplyBlock& b = v[first + i];
+    f += 4.0 * b.len * b.k * std::max(0, b.hi - b.lo);
+  }
+  return f;
+}
+
+struct Timer {
+  paraht_ctx* c;
+  bool on;
+  std::vector<hipEvent_t> ev;
+  explicit Timer(paraht_ctx* cx, bool enable) : c(cx), on(enable) {}
+  ~Timer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  int mark() {
+    if (!on) return -1;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    if (hipEventRecord(e, c->stream) != hipSuccess) return -1;
+    ev.push_back(e);
+    return (int)ev.size() - 1;
+  }
+  double ms(int a, int b) {
+    if (!on || a < 0 || b < 0) return 0.0;
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) return 0.0;
+    return t;
+  }
+};
+
+// ------------------------------------------------------------------------
+// Stage 1
+// ------------------------------------------------------------------------
+int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
+           double* Q, int ldq, double* Z, int ldz, int nb, int p, int flags) {
+  if (n <= nb + 1) return 0;  // no panel: ab2rht.f90:71 loop is empty
+  hipStream_t st = c->stream;
+  const int P = p * nb;
+  int nbmax = (n - nb) / ((p - 1) * nb);
+  if ((n - nb) % ((p - 1) * nb)) ++nbmax;
+
+  // block descriptors of every panel, uploaded once
+  struct PanelInfo {
+    int jb0, nblocks, first, i0top;
+  };
+  std::vector<PanelInfo> panels;
+  std::vector<ApplyBlock> BL, AL, QL, RR;  // RR: right blocks (B/A/Z differ only in lo/hi)
+  for (int jb = 1; jb <= n - nb - 1; jb += nb) {
+    int nblocks = (n - nb - jb + 1) / ((p - 1) * nb);
+    if ((n - nb - jb + 1) % ((p - 1) * nb)) ++nblocks;
+    PanelInfo pi{jb - 1, nblocks, (int)BL.size(), jb - 1 + nb};
+    for (int bl = 0; bl < nblocks; ++bl) {
+      const int i0 = jb - 1 + nb + (nblocks - 1 - bl) * (p - 1) * nb;
+      const int len = std::min(n - i0, P);
+      const int k = std::min(nb, len);
+      const int kr = (len > nb) ? nb : len;
+      const long long voff = (long long)bl * P * nb, toff = (long long)bl * nb * nb;
+      BL.push_back({i0, len, k, i0, n, 0, voff, toff});
+      AL.push_back({i0, len, k, jb - 1 + nb, n, 0, voff, toff});
+      QL.push_back({i0, len, k, 0, n, 0, voff, toff});
+      RR.push_back({i0, len, kr, 0, n, 0, voff, toff});
+    }
+    panels.push_back(pi);
+  }
+  const size_t nd = BL.size();
+  // B right blocks are applied to rows [0, i0+len)
+  std::vector<ApplyBlock> BR = RR;
+  for (auto& b : BR) b.hi = b.start + b.len;
+
+  const size_t vsz = (size_t)nbmax * P * nb, tsz = (size_t)nbmax * nb * nb;
+  const size_t scr = (size_t)P * P + (size_t)nb * P;
+  const size_t dsz = 5 * nd * sizeof(ApplyBlock);
+  size_t bytes = (2 * vsz + 2 * tsz + scr) * sizeof(double) + dsz + 256;
+  if (int e = ensure_ws(c, bytes)) return e;
+  double* VL = (double*)c->ws;
+  double* VR = VL + vsz;
+  double* TL = VR + vsz;
+  double* TR = TL + tsz;
+  double* scratch = TR + tsz;
+  ApplyBlock* dD = (ApplyBlock*)(((uintptr_t)(scratch + scr) + 255) & ~(uintptr_t)255);
+  ApplyBlock* dBL = dD;
+  ApplyBlock* dAL = dBL + nd;
+  ApplyBlock* dQL = dAL + nd;
+  ApplyBlock* dRR = dQL + nd;
+  ApplyBlock* dBR = dRR + nd;
+  CK(hipMemcpyAsync(dBL, BL.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(dAL, AL.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(dQL, QL.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(dRR, RR.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(dBR, BR.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
+  // the host vectors die with this frame
+  CK(hipStreamSynchronize(st));
+
+  const bool tc = (flags & PARAHT_TIME_COMPONENTS) != 0;
+  Timer tm(c, tc);
+  std::vector<int> marks;
+  for (const PanelInfo& pi : panels) {
+    const int m0 = tm.mark();
+    PROF(PARAHT_K_S1_PANEL_QR, 0.0, 0.0,
+         pht::s1_panel_qr(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
+                          (long long)P * nb, TL, nb, (long long)nb * nb));
+    const int m1 = tm.mark();
+    PROF(PARAHT_K_WY_APPLY, wy_flops(BL, pi.first, pi.nblocks), 0.0,
+         pht::apply_blocks(st, pht::kLeftT, B, ldb, VL, P, TL, nb, dBL + pi.first,
+                           pi.nblocks, nb, pi.i0top, n));
+    PROF(PARAHT_K_WY_APPLY, wy_flops(AL, pi.first, pi.nblocks), 0.0,
+         pht::apply_blocks(st, pht::kLeftT, A, lda, VL, P, TL, nb, dAL + pi.first,
+                           pi.nblocks, nb, pi.jb0 + nb, n));
+    PROF(PARAHT_K_WY_APPLY, wy_flops(QL, pi.first, pi.nblocks), 0.0,
+         pht::apply_blocks(st, pht::kRightN, Q, ldq, VL, P, TL, nb, dQL + pi.first,
+                           pi.nblocks, nb, 0, n));
+    const int m2 = tm.mark();
+    for (int bl = 0; bl < pi.nblocks; ++bl) {
+      const ApplyBlock& b = BR[pi.first + bl];
+      PROF(PARAHT_K_S1_RQ_GEN, 0.0, 0.0,
+           pht::s1_rq_gen(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
+                          TR + (size_t)bl * nb * nb, nb, scratch));
+      PROF(PARAHT_K_WY_APPLY, wy_flops(BR, pi.first + bl, 1), 0.0,
+           pht::apply_blocks(st, pht::kRightN, B, ldb, VR, P, TR, nb,
+                             dBR + pi.first + bl, 1, b.k, 0, b.start + b.len));
+      PROF(PARAHT_K_OTHER, 0.0, 0.0,
+           pht::zero_below(st, B, ldb, b.start, b.len, b.len > nb ? nb : b.len));
+    }
+    const int m3 = tm.mark();
+    PROF(PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
+         pht::apply_blocks(st, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
+                           pi.nblocks, nb, 0, n));
+    PROF(PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
+         pht::apply_blocks(st, pht::kRightN, Z, ldz, VR, P, TR, nb, dRR + pi.first,
+                           pi.nblocks, nb, 0, n));
+    const int m4 = tm.mark();
+    marks.insert(marks.end(), {m0, m1, m2, m3, m4});
+  }
+  if (tc) {
+    CK(hipStreamSynchronize(st));
+    double gl = 0, gr = 0, mul = 0;
+    for (size_t i = 0; i < marks.size(); i += 5) {
+      gl += tm.ms(marks[i], marks[i + 1]);
+      mul += tm.ms(marks[i + 1], marks[i + 2]) + tm.ms(marks[i + 3], marks[i + 4]);
+      gr += tm.ms(marks[i + 2], marks[i + 3]);
+    }
+    c->times[1] = gl;
+    c->times[2] = gr;
+    // same line layout as ab2rht.f90:488-489 (seconds)
+    printf("      genlefttime: %16.6f ;genrighttime: %16.6f ;multime: %16.6f\n",
+           gl * 1e-3, gr * 1e-3, mul * 1e-3);
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------
+// Stage 2 (Givens)
+// ------------------------------------------------------------------------
+int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
+           double* Q, int ldq, double* Z, int ldz, int r, int nq, int flags) {
+  if (n <= 2) return 0;  // rht2ht.f90:1417 loop is empty
+  if (r > 64) return PARAHT_ERR_UNSUPPORTED;
+  hipStream_t st = c->stream;
+  const int ldr = r + nq;
+  if (ldr > 160) return PARAHT_ERR_UNSUPPORTED;  // LDS windows of the sweeps
+  int nbl = (n - 1) / r + ((n - 1) % r != 0);
+  const size_t tsz = (size_t)ldr * nq * nbl;
+  const size_t msz = (size_t)ldr * ldr * nbl;
+  if (int e = ensure_ws(c, (4 * tsz + msz) * sizeof(double))) return e;
+  double* CU = (double*)c->ws;
+  double* SU = CU + tsz;
+  double* CV = SU + tsz;
+  double* SV = CV + tsz;
+  const bool tc = (flags & PARAHT_TIME_COMPONENTS) != 0;
+  Timer tm(c, tc);
+  std::vector<int> marks;
+  for (int s = 1; s <= n - 2; s += nq) {
+    pht::S2Params P;
+    P.n = n;
+    P.r = r;
+    P.nq = nq;
+    P.jblock = s;
+    P.nblocks = (n - s) / r + ((n - s) % r != 0);
+    P.ldr = ldr;
+    P.M = SV + tsz;
+    const int m0 = tm.mark();
+    PROF(PARAHT_K_S2_GEN, 0.0, 0.0,
+         pht::s2_generate(st, A, lda, B, ldb, P, CU, SU, CV, SV));
+    const int m1 = tm.mark();
+    // algorithmic traffic: every element of the sweep's columns s+1..n of
+    // Q and Z, and of A and B above the window, is read and written once
+    const double w = (double)(n - s);
+    const double rows_ab = std::min<double>(n, s + (double)(P.nblocks - 1) * r);
+    PROF(PARAHT_K_S2_ROWS, 0.0, 16.0 * w * (2.0 * n + 2.0 * rows_ab),
+         pht::s2_apply_rows(st, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV));
+    PROF(PARAHT_K_S2_COLS, 0.0, 0.0, pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU));
+    const int m2 = tm.mark();
+    marks.insert(marks.end(), {m0, m1, m2});
+  }
+  if (tc) {
+    CK(hipStreamSynchronize(st));
+    double gen = 0, app = 0;
+    for (size_t i = 0; i < marks.size(); i += 3) {
+      gen += tm.ms(marks[i], marks[i + 1]);
+      app += tm.ms(marks[i + 1], marks[i + 2]);
+    }
+    c->times[4] = gen;
+    // same line layout as rht2ht.f90:2045-2046 (seconds)
+    printf("      gen: %16.6f ;lookahead: %16.6f ;application: %16.6f\n",
+           gen * 1e-3, 0.0, app * 1e-3);
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------
+std::mutex g_mu;
+paraht_ctx* g_default[64] = {nullptr};
+
+int default_ctx(paraht_ctx** out) {
+  int dev = 0;
+  CK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return PARAHT_ERR_NODEV;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_default[dev]) {
+    int e = paraht_ctx_create(&g_default[dev], dev);
+    if (e) return e;
+  }
+  *out = g_default[dev];
+  return 0;
+}
+
+struct DevMat {
+  double* d = nullptr;
+  ~DevMat() {
+    if (d) (void)hipFree(d);
+  }
+};
+
+// copy an n x n host matrix (leading dimension ld) to a dense device matrix
+int upload(hipStream_t st, DevMat& m, const double* h, int n, int ld) {
+  CK(hipMalloc(&m.d, std::max<size_t>(1, (size_t)n * n) * sizeof(double)));
+  if (n > 0)
+    CK(hipMemcpy2DAsync(m.d, (size_t)n * sizeof(double), h, (size_t)ld * sizeof(double),
+                        (size_t)n * sizeof(double), n, hipMemcpyHostToDevice, st));
+  return 0;
+}
+
+int download(hipStream_t st, double* h, int ld, const DevMat& m, int n) {
+  if (n > 0)
+    CK(hipMemcpy2DAsync(h, (size_t)ld * sizeof(double), m.d, (size_t)n * sizeof(double),
+                        (size_t)n * sizeof(double), n, hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+enum Which { kStage1 = 1, kStage2 = 2, kBoth = 3 };
+
+int run_dev(paraht_ctx* c, Which w, int n, double* A, int lda, double* B,
+            int ldb, double* Q, int ldq, double* Z, int ldz, int r, int p,
+            int nq, int flags) {
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  CK(hipSetDevice(c->device));
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  CK(hipEventCreate(&e2));
+  CK(hipEventRecord(e0, c->stream));
+  int st = 0;
+  if (w & kStage1) st = stage1(c, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, p, flags);
+  if (!st) {
+    hipError_t he = hipEventRecord(e1, c->stream);
+    if (he != hipSuccess) st = hip_fail(he, "hipEventRecord", __LINE__);
+  }
+  if (!st && (w & kStage2))
+    st = stage2(c, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, nq, flags);
+  if (!st) {
+    hipError_t he = hipEventRecord(e2, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) st = hip_fail(he, "synchronize", __LINE__);
+  }
+  if (!st) {
+    float t1 = 0.f, t2 = 0.f;
+    if (hipEventElapsedTime(&t1, e0, e1) == hipSuccess) c->times[0] = t1;
+    if (hipEventElapsedTime(&t2, e1, e2) == hipSuccess) c->times[3] = t2;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  return st;
+}
+
+int run_host(Which w, int n, double* A, int lda, double* B, int ldb, double* Q,
+             int ldq, double* Z, int ldz, int r, int p, int nq, int flags) {
+  paraht_ctx* c = nullptr;
+  if (int e = default_ctx(&c)) return e;
+  if (n == 0) return 0;
+  DevMat dA, dB, dQ, dZ;
+  if (int e = upload(c->stream, dA, A, n, lda)) return e;
+  if (int e = upload(c->stream, dB, B, n, ldb)) return e;
+  if (int e = upload(c->stream, dQ, Q, n, ldq)) return e;
+  if (int e = upload(c->stream, dZ, Z, n, ldz)) return e;
+  int st = run_dev(c, w, n, dA.d, n, dB.d, n, dQ.d, n, dZ.d, n, r, p, nq, flags);
+  if (st) return st;
+  if (int e = download(c->stream, A, lda, dA, n)) return e;
+  if (int e = download(c->stream, B, ldb, dB, n)) return e;
+  if (int e = download(c->stream, Q, ldq, dQ, n)) return e;
+  if (int e = download(c->stream, Z, ldz, dZ, n)) return e;
+  CK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// C-ABI
+// ------------------------------------------------------------------------
+extern "C" {
+
+int paraht_ctx_create(paraht_ctx** ctx, int device) {
+  if (!ctx) return -1;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PARAHT_ERR_NODEV;
+  if (device < 0 || device >= ndev) return -2;
+  hipDeviceProp_t prop;
+  CK(hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    if (verbose()) fprintf(stderr, "paraht: device %d is %s, not gfx950\n", device, prop.gcnArchName);
+    return PARAHT_ERR_NODEV;
+  }
+  CK(hipSetDevice(device));
+  paraht_ctx* c = new paraht_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreate", __LINE__);
+  }
+  c->stream = c->own;
+  *ctx = c;
+  return 0;
+}
+
+int paraht_ctx_destroy(paraht_ctx* ctx) {
+  if (!ctx) return -1;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->own) (void)hipStreamSynchronize(ctx->own);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
+  if (ctx->own) (void)hipStreamDestroy(ctx->own);
+  delete ctx;
+  return 0;
+}
+
+int paraht_ctx_set_stream(paraht_ctx* ctx, void* stream) {
+  if (!ctx) return -1;
+  ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+  return 0;
+}
+
+int paraht_ctx_last_times(paraht_ctx* ctx, double* ms, int nmax) {
+  if (!ctx) return -1;
+  if (!ms || nmax < 0) return -2;
+  int k = std::min(nmax, 5);
+  for (int i = 0; i < k; ++i) ms[i] = ctx->times[i];
+  return k;
+}
+
+int paraht_ctx_profile(paraht_ctx* ctx, int enable) {
+  if (!ctx) return -1;
+  if (int e = prof_flush(ctx)) return e;
+  for (int k = 0; k < PARAHT_NKERNELS; ++k) {
+    ctx->k_ms[k] = ctx->k_flops[k] = ctx->k_bytes[k] = 0.0;
+    ctx->k_n[k] = 0;
+  }
+  ctx->profiling = enable != 0;
+  return 0;
+}
+
+int paraht_ctx_kernel_stats(paraht_ctx* ctx, int kernel, double* total_ms,
+                            long long* launches, double* flops, double* bytes) {
+  if (!ctx) return -1;
+  if (kernel < 0 || kernel >= PARAHT_NKERNELS) return -2;
+  if (int e = prof_flush(ctx)) return e;
+  if (total_ms) *total_ms = ctx->k_ms[kernel];
+  if (launches) *launches = ctx->k_n[kernel];
+  if (flops) *flops = ctx->k_flops[kernel];
+  if (bytes) *bytes = ctx->k_bytes[kernel];
+  return 0;
+}
+
+int paraht_ab2rht(int n, double* A, int lda, double* B, int ldb, double* Q,
+                  int ldq, double* Z, int ldz, int nb, int p, int flags) {
+  if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
+  if (nb < 1) return -10;
+  if (p < 2) return -11;
+  return run_host(kStage1, n, A, lda, B, ldb, Q, ldq, Z, ldz, nb, p, 1, flags);
+}
+
+int paraht_rht2ht(int n, double* A, int lda, double* B, int ldb, double* Q,
+                  int ldq, double* Z, int ldz, int r, int nq, int flags) {
+  if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
+  if (r < 1) return -10;
+  if (nq < 1) return -11;
+  return run_host(kStage2, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, 2, nq, flags);
+}
+
+int paraht_rht2ht_unblocked(int n, double* A, int lda, double* B, int ldb,
+                            double* Q, int ldq, double* Z, int ldz, int r) {
+  if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
+  if (r < 1) return -10;
+  // one column per sweep: every rotation is applied before the next column
+  return run_host(kStage2, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, 2, 1, 0);
+}
+
+int paraht_reduce(int n, double* A, int lda, double* B, int ldb, double* Q,
+                  int ldq, double* Z, int ldz, int r, int p, int nq, int flags) {
+  if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
+  if (r < 1) return -10;
+  if (p < 2) return -11;
+  if (nq < 1) return -12;
+  return run_host(kBoth, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, p, nq, flags);
+}
+
+int paraht_ab2rht_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
+                      int ldb, double* dQ, int ldq, double* dZ, int ldz,
+                      int nb, int p, int flags) {
+  if (!ctx) return -1;
+  if (int e = check_common(n, lda, ldb, ldq, ldz)) return e - 1;
+  if (nb < 1) return -11;
+  if (p < 2) return -12;
+  return run_dev(ctx, kStage1, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, nb, p, 1, flags);
+}
+
+int paraht_rht2ht_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
+                      int ldb, double* dQ, int ldq, double* dZ, int ldz, int r,
+                      int nq, int flags) {
+  if (!ctx) return -1;
+  if (int e = check_common(n, lda, ldb, ldq, ldz)) return e - 1;
+  if (r < 1) return -11;
+  if (nq < 1) return -12;
+  return run_dev(ctx, kStage2, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, 2, nq, flags);
+}
+
+int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
+                      int ldb, double* dQ, int ldq, double* dZ, int ldz, int r,
+                      int p, int nq, int flags) {
+  if (!ctx) return -1;
+  if (int e = check_common(n, lda, ldb, ldq, ldz)) return e - 1;
+  if (r < 1) return -11;
+  if (p < 2) return -12;
+  if (nq < 1) return -13;
+  return run_dev(ctx, kBoth, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, p, nq, flags);
+}
+
+int paraht_version(void) { return 100; }
+
+const char* paraht_status_string(int status) {
+  if (status == 0) return "success";
+  if (status < 0) return "illegal argument";
+  switch (status) {
+    case PARAHT_ERR_HIP: return "HIP runtime error";
+    case PARAHT_ERR_NOMEM: return "out of memory";
+    case PARAHT_ERR_NODEV: return "no usable gfx950 device";
+    case PARAHT_ERR_UNSUPPORTED: return "unsupported parameters";
+    default: return "unknown status";
+  }
+}
+
+}  // extern "C"
