@@ -51,6 +51,8 @@ struct S2Params {
   int nblocks;           // bulge blocks of this sweep
   int ldr;               // r + nq
   double* M;             // nblocks x ldr x ldr workspace (deferred rotations)
+  int* progress;         // nq step counters of the sweep's workgroups
+  int* errflag;          // set when a workgroup's wait hit the spin limit
 };
 hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
                        const S2Params& P, double* CU, double* SU, double* CV,

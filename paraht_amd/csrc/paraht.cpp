@@ -111,19 +111,21 @@ int prof_flush(paraht_ctx* c) {
   return 0;
 }
 
-int prof_event(paraht_ctx* c, int* idx) {
-  if (c->pool_used + 2 > c->pool.size()) {
-    if (c->pool.size() >= 8192) {
-      if (int e = prof_flush(c)) return e;
-    } else {
-      size_t grow = c->pool.size() ? c->pool.size() : 512;
-      for (size_t i = 0; i < grow; ++i) {
-        hipEvent_t ev;
-        CK(hipEventCreate(&ev));
-        c->pool.push_back(ev);
-      }
-    }
+// Makes room for `need` more events (flushing finished records if the pool
+// is at its cap), so both events of one launch come from the same pool round.
+int prof_reserve(paraht_ctx* c, size_t need) {
+  if (c->pool_used + need <= c->pool.size()) return 0;
+  if (c->pool.size() >= 8192) return prof_flush(c);
+  size_t grow = c->pool.size() ? c->pool.size() : 512;
+  for (size_t i = 0; i < grow; ++i) {
+    hipEvent_t ev;
+    CK(hipEventCreate(&ev));
+    c->pool.push_back(ev);
   }
+  return 0;
+}
+
+int prof_event(paraht_ctx* c, int* idx) {
   *idx = (int)c->pool_used++;
   CK(hipEventRecord(c->pool[*idx], c->stream));
   return 0;
@@ -134,7 +136,11 @@ struct Prof {
   int kid, e0 = -1;
   double flops, bytes;
   Prof(paraht_ctx* cx, int k, double f, double b) : c(cx), kid(k), flops(f), bytes(b) {}
-  int begin() { return c->profiling ? prof_event(c, &e0) : 0; }
+  int begin() {
+    if (!c->profiling) return 0;
+    if (int e = prof_reserve(c, 2)) return e;
+    return prof_event(c, &e0);
+  }
   int end() {
     if (!c->profiling) return 0;
     int e1 = -1;
@@ -320,7 +326,8 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   int nbl = (n - 1) / r + ((n - 1) % r != 0);
   const size_t tsz = (size_t)ldr * nq * nbl;
   const size_t msz = (size_t)ldr * ldr * nbl;
-  if (int e = ensure_ws(c, (4 * tsz + msz) * sizeof(double))) return e;
+  if (int e = ensure_ws(c, (4 * tsz + msz) * sizeof(double) + (nq + 64) * sizeof(int)))
+    return e;
   double* CU = (double*)c->ws;
   double* SU = CU + tsz;
   double* CV = SU + tsz;
@@ -337,6 +344,8 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     P.nblocks = (n - s) / r + ((n - s) % r != 0);
     P.ldr = ldr;
     P.M = SV + tsz;
+    P.progress = (int*)(P.M + msz);
+    P.errflag = P.progress + nq;
     const int m0 = tm.mark();
     PROF(PARAHT_K_S2_GEN, 0.0, 0.0,
          pht::s2_generate(st, A, lda, B, ldb, P, CU, SU, CV, SV));
@@ -350,6 +359,15 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     PROF(PARAHT_K_S2_COLS, 0.0, 0.0, pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU));
     const int m2 = tm.mark();
     marks.insert(marks.end(), {m0, m1, m2});
+  }
+  {
+    int err = 0;
+    CK(hipMemcpyAsync(&err, (int*)(SV + tsz + msz) + nq, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (err) {
+      if (verbose()) fprintf(stderr, "paraht: stage-2 sweep wait exceeded its spin limit\n");
+      return PARAHT_ERR_HIP;
+    }
   }
   if (tc) {
     CK(hipStreamSynchronize(st));
