@@ -257,13 +257,20 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   CK(hipStreamSynchronize(st));
 
   const bool tc = (flags & PARAHT_TIME_COMPONENTS) != 0;
+  // register-resident generators cover blocks of <= 128 rows, <= 64 reflectors
+  const bool fast_gen = nb <= 64 && P <= 128 && getenv("PARAHT_GENERIC_S1") == nullptr;
   Timer tm(c, tc);
   std::vector<int> marks;
   for (const PanelInfo& pi : panels) {
     const int m0 = tm.mark();
-    PROF(PARAHT_K_S1_PANEL_QR, 0.0, 0.0,
-         pht::s1_panel_qr(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
-                          (long long)P * nb, TL, nb, (long long)nb * nb));
+    if (fast_gen)
+      PROF(PARAHT_K_S1_PANEL_QR, 0.0, 0.0,
+           pht::s1_panel_qr_fast(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
+                                 (long long)P * nb, TL, nb, (long long)nb * nb));
+    else
+      PROF(PARAHT_K_S1_PANEL_QR, 0.0, 0.0,
+           pht::s1_panel_qr(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
+                            (long long)P * nb, TL, nb, (long long)nb * nb));
     const int m1 = tm.mark();
     PROF(PARAHT_K_WY_APPLY, wy_flops(BL, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kLeftT, B, ldb, VL, P, TL, nb, dBL + pi.first,
@@ -277,9 +284,14 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     const int m2 = tm.mark();
     for (int bl = 0; bl < pi.nblocks; ++bl) {
       const ApplyBlock& b = BR[pi.first + bl];
-      PROF(PARAHT_K_S1_RQ_GEN, 0.0, 0.0,
-           pht::s1_rq_gen(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
-                          TR + (size_t)bl * nb * nb, nb, scratch));
+      if (fast_gen && b.len > nb)
+        PROF(PARAHT_K_S1_RQ_GEN, 0.0, 0.0,
+             pht::s1_rq_fast(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
+                             TR + (size_t)bl * nb * nb, nb));
+      else
+        PROF(PARAHT_K_S1_RQ_GEN, 0.0, 0.0,
+             pht::s1_rq_gen(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
+                            TR + (size_t)bl * nb * nb, nb, scratch));
       PROF(PARAHT_K_WY_APPLY, wy_flops(BR, pi.first + bl, 1), 0.0,
            pht::apply_blocks(st, pht::kRightN, B, ldb, VR, P, TR, nb,
                              dBR + pi.first + bl, 1, b.k, 0, b.start + b.len));
