@@ -25,6 +25,35 @@ METRIC = "n=8000 FP64 HT-reduce s & %FP64 peak; batched pencils/s; fwd+bwd/fwd r
 FP64_PEAK = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
 HBM_PEAK = 8.0e12    # bytes/s
 
+# kernel classes timed by the library -> (kernel, roofline bound).  The FP64
+# vector and FP64 matrix peaks of MI355X are the same 78.6 TFLOP/s, so the
+# generators (vector FP64) and the compact-WY applier (FP64 MFMA) share one
+# compute roofline.
+KERNEL_CLASSES = {
+    "wy_apply": ("k_apply_blocks (compact-WY block reflectors, FP64 MFMA)", "mfma"),
+    "s1_panel_qr": ("k_s1_panel_qr_fast (panel QR + T)", "mfma"),
+    "s1_rq_gen": ("k_s1_rq_fast (RQ + LQ right generator + T)", "mfma"),
+    "s2_gen": ("k_s2_sweep (Givens bulge chase of one sweep)", "mfma"),
+    "s2_rows": ("k_s2_rows_v (rotations on rows of Q, Z and of A, B above the window)", "hbm"),
+    "s2_cols": ("k_s2_cols_v (rotations on columns of A, B right of the window)", "hbm"),
+}
+COPY_CAL_DOUBLES = 1 << 27  # 1 GiB read + 1 GiB written per calibration launch
+
+
+def traffic_per_launch(cls, args):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the copy-kernel
+    calibration), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles",
+                        f"traffic_{cls}_n{args.n}_r{args.r}_p{args.p}_nq{args.nq}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_dispatch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
 
 def cpu_baseline(args):
     """The oracle's restatement of ParaHT (stage 1 + the Householder stage 2
@@ -58,6 +87,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=1536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="before the warmup, launch a plain elementwise copy of "
+                         "1 GiB three times (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the HT reduction of one pencil is "
@@ -88,6 +121,16 @@ def main():
         Z.copy_(eye)
         torch.cuda.synchronize()
         ph.reduce(A, B, r, p, nq, Q, Z, ctx=ctx)
+
+    if args.copy_calibration:
+        src = torch.ones(COPY_CAL_DOUBLES, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(f"copy calibration: 3 launches of {COPY_CAL_DOUBLES * 8} bytes read + "
+              f"{COPY_CAL_DOUBLES * 8} bytes written", file=sys.stderr, flush=True)
+        del src, dst
 
     for i in range(args.warmup):
         step()
@@ -138,25 +181,25 @@ def main():
     if kstats:
         tot = sum(v[0] for v in kstats.values()) or 1.0
         out["kernels"] = {k: {"ms": v[0] / args.steps, "launches": v[1] // args.steps,
-                              "share": v[0] / tot} for k, v in kstats.items() if v[1]}
+                              "avg_us": 1e3 * v[0] / v[1], "share": v[0] / tot}
+                          for k, v in kstats.items() if v[1]}
         dom = max(kstats, key=lambda k: kstats[k][0])
         out["dominant_kernel"] = dom
-        # roofline of the dominant throughput kernel (the generation kernels
-        # are latency-bound chains and have no throughput roofline)
-        cands = {k: v for k, v in kstats.items() if k in ("wy_apply", "s2_rows") and v[1]}
-        if cands:
-            k = max(cands, key=lambda x: cands[x][0])
-            ms_k, cnt, fl, by = cands[k]
-            if k == "wy_apply":
-                ach = fl / cnt / (ms_k / cnt / 1e3) / 1e12
-                out["roofline"] = {"kernel": "k_apply_blocks (FP64 MFMA compact-WY)",
-                                   "bound": "mfma", "achieved": ach, "peak": 78.6,
-                                   "unit": "TFLOP/s", "frac": ach / 78.6, "traffic": None}
+        if dom in KERNEL_CLASSES:
+            ms_k, cnt, fl, by = kstats[dom]
+            name, bound = KERNEL_CLASSES[dom]
+            t_launch = ms_k / cnt / 1e3  # average launch duration (HIP events), s
+            if bound == "mfma":
+                ach = fl / cnt / t_launch / 1e12
+                peak, unit = FP64_PEAK / 1e12, "TFLOP/s"
             else:
-                ach = by / cnt / (ms_k / cnt / 1e3) / 1e9
-                out["roofline"] = {"kernel": "k_s2_rows (rotation sweeps of Q, Z, A, B)",
-                                   "bound": "hbm", "achieved": ach, "peak": 8000.0,
-                                   "unit": "GB/s", "frac": ach / 8000.0, "traffic": None}
+                ach = by / cnt / t_launch / 1e9
+                peak, unit = HBM_PEAK / 1e9, "GB/s"
+            out["roofline"] = {"kernel": name, "bound": bound, "achieved": ach,
+                               "peak": peak, "unit": unit, "frac": ach / peak,
+                               "traffic": traffic_per_launch(dom, args),
+                               "avg_launch_us": t_launch * 1e6, "launches": cnt // args.steps,
+                               "algorithmic_per_launch": (fl if bound == "mfma" else by) / cnt}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

@@ -61,6 +61,10 @@ struct S2Params {
   int* progress;         // nq step counters of the sweep's workgroups
   int* errflag;          // set when a workgroup's wait hit the spin limit
 };
+// Resets the sweep's tape to identity rotations, M to identity and the
+// progress counters; must precede s2_generate.
+hipError_t s2_prepare(hipStream_t st, const S2Params& P, double* CU, double* SU,
+                      double* CV, double* SV);
 hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
                        const S2Params& P, double* CU, double* SU, double* CV,
                        double* SV);

@@ -169,6 +169,63 @@ double wy_flops(const std::vector<ApplyBlock>& v, size_t first, int cnt) {
   return f;
 }
 
+// algorithmic flops of the stage-1 generators (leading terms of the LAPACK
+// counts of the routines they restate)
+double qr_flops(int m, int k) {  // DGEQR2 (m x k) + DLARFT
+  return 2.0 * m * k * k - (2.0 / 3.0) * k * k * k + (double)m * k * k;
+}
+double rq_flops(int len, int nb) {  // DGERQ2 + first nb rows of Q + DGELQ2 + DLARFT
+  const double l = len;
+  return (4.0 / 3.0) * l * l * l + 2.0 * nb * l * l + 2.0 * l * nb * nb -
+         (2.0 / 3.0) * nb * nb * nb + l * nb * nb;
+}
+
+// algorithmic flops of one stage-2 sweep's generation (6 per rotation and
+// element pair, 2 per mat-vec term); mirrors the step loop of k_s2_sweep
+double s2_gen_flops(int n, int r, int nq, int s) {
+  const int ldr = r + nq;
+  double f = 0.0;
+  for (int ic = 1; ic <= nq; ++ic) {
+    const int jc = s + ic - 1;
+    for (int ib = 1;; ++ib) {
+      int col, row0;
+      if (ib == 1) {
+        if (jc > n) break;
+        col = jc;
+        row0 = jc + 1;
+      } else {
+        col = jc + 1 + (ib - 2) * r;
+        if (col > n) break;
+        row0 = col + r;
+      }
+      const int nb = std::min(r, n - row0 + 1);
+      const int L = std::min(ldr - 1, n - (s + (ib - 1) * r));
+      if (ic > 1 && L > 0) f += 2.0 * L * L * (row0 + r - 1 <= n ? 2 : 1);
+      if (nb >= 2) {
+        const int istart = std::max(s + 1, s + 1 + (ib - 1 - nq + ic) * r);
+        const int nAr = std::min(n, row0 + r + nb - 1) - istart + 1;
+        const int nBr = std::max(0, row0 - istart);
+        f += 6.0 * (nb - 1) * (2.0 * nb + ldr + nAr + nBr);
+      }
+    }
+  }
+  return f;
+}
+
+// algorithmic bytes of one sweep's column updates: each block's rows of A
+// and B right of the window, read and written once
+double s2_cols_bytes(int n, int r, int nq, int s, int nblocks) {
+  double b = 0.0;
+  for (int ib = 1; ib <= nblocks; ++ib) {
+    const int W = std::min(r + nq - 1, n - (s + (ib - 1) * r));
+    if (W <= 0) continue;
+    const int istopA = (ib == 1) ? s + nq - 1 : s + (ib - 2) * r + nq;
+    const int istopB = s + (ib - 1) * r + r + nq - 1;
+    b += 16.0 * W * (std::max(0, n - istopA) + std::max(0, n - istopB));
+  }
+  return b;
+}
+
 struct Timer {
   paraht_ctx* c;
   bool on;
@@ -263,12 +320,14 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   std::vector<int> marks;
   for (const PanelInfo& pi : panels) {
     const int m0 = tm.mark();
+    double qf = 0.0;
+    for (int bl = 0; bl < pi.nblocks; ++bl) qf += qr_flops(BL[pi.first + bl].len, BL[pi.first + bl].k);
     if (fast_gen)
-      PROF(PARAHT_K_S1_PANEL_QR, 0.0, 0.0,
+      PROF(PARAHT_K_S1_PANEL_QR, qf, 0.0,
            pht::s1_panel_qr_fast(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
                                  (long long)P * nb, TL, nb, (long long)nb * nb));
     else
-      PROF(PARAHT_K_S1_PANEL_QR, 0.0, 0.0,
+      PROF(PARAHT_K_S1_PANEL_QR, qf, 0.0,
            pht::s1_panel_qr(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
                             (long long)P * nb, TL, nb, (long long)nb * nb));
     const int m1 = tm.mark();
@@ -284,12 +343,13 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     const int m2 = tm.mark();
     for (int bl = 0; bl < pi.nblocks; ++bl) {
       const ApplyBlock& b = BR[pi.first + bl];
+      const double rf = b.len > nb ? rq_flops(b.len, nb) : (4.0 / 3.0) * b.len * b.len * b.len;
       if (fast_gen && b.len > nb)
-        PROF(PARAHT_K_S1_RQ_GEN, 0.0, 0.0,
+        PROF(PARAHT_K_S1_RQ_GEN, rf, 0.0,
              pht::s1_rq_fast(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
                              TR + (size_t)bl * nb * nb, nb));
       else
-        PROF(PARAHT_K_S1_RQ_GEN, 0.0, 0.0,
+        PROF(PARAHT_K_S1_RQ_GEN, rf, 0.0,
              pht::s1_rq_gen(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
                             TR + (size_t)bl * nb * nb, nb, scratch));
       PROF(PARAHT_K_WY_APPLY, wy_flops(BR, pi.first + bl, 1), 0.0,
@@ -359,7 +419,8 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     P.progress = (int*)(P.M + msz);
     P.errflag = P.progress + nq;
     const int m0 = tm.mark();
-    PROF(PARAHT_K_S2_GEN, 0.0, 0.0,
+    PROF(PARAHT_K_OTHER, 0.0, 0.0, pht::s2_prepare(st, P, CU, SU, CV, SV));
+    PROF(PARAHT_K_S2_GEN, s2_gen_flops(n, r, nq, s), 0.0,
          pht::s2_generate(st, A, lda, B, ldb, P, CU, SU, CV, SV));
     const int m1 = tm.mark();
     // algorithmic traffic: every element of the sweep's columns s+1..n of
@@ -368,7 +429,8 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     const double rows_ab = std::min<double>(n, s + (double)(P.nblocks - 1) * r);
     PROF(PARAHT_K_S2_ROWS, 0.0, 16.0 * w * (2.0 * n + 2.0 * rows_ab),
          pht::s2_apply_rows(st, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV));
-    PROF(PARAHT_K_S2_COLS, 0.0, 0.0, pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU));
+    PROF(PARAHT_K_S2_COLS, 0.0, s2_cols_bytes(n, r, nq, s, P.nblocks),
+         pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU));
     const int m2 = tm.mark();
     marks.insert(marks.end(), {m0, m1, m2});
   }

@@ -590,14 +590,17 @@ __global__ __launch_bounds__(64) void k_s2_cols_v(
   X(64, 8) X(64, 16) X(64, 32)
 
 // ---------------------------------------------------------------------------
-hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
-                       const S2Params& P, double* CU, double* SU, double* CV,
-                       double* SV) {
+hipError_t s2_prepare(hipStream_t st, const S2Params& P, double* CU, double* SU,
+                      double* CV, double* SV) {
   const size_t tsz = (size_t)P.ldr * P.nq * P.nblocks;
   hipLaunchKernelGGL(k_s2_prep, dim3(128), dim3(256), 0, st, CU, SU, CV, SV, tsz,
                      P.M, P.ldr, P.nblocks, P.progress, P.nq, P.errflag);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
+                       const S2Params& P, double* CU, double* SU, double* CV,
+                       double* SV) {
   const dim3 g(P.nq), b(kGenThreads);
 #define PHT_GEN(R)                                                              \
   hipLaunchKernelGGL(k_s2_sweep<R>, g, b, 0, st, A, lda, B, ldb, P.n, P.r, P.nq, \
