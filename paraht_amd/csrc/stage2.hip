@@ -62,6 +62,8 @@ __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 // 887-909) are applied as one mat-vec with it.
 constexpr int kSpinLimit = 1 << 26;
 constexpr int kDone = 1 << 29;
+constexpr int kMaxLdr = 160;                 // r + nq (host checks)
+constexpr int kMaxHalf = (kMaxLdr + 1) / 2;  // reduction half of the mat-vec
 
 __global__ void k_s2_prep(double* CU, double* SU, double* CV, double* SV,
                           size_t tsz, double* Mw, int ldr, int nblocks,
@@ -110,6 +112,7 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
   __shared__ double Bb[RMAX][RMAX + 1];
   __shared__ double lc[RMAX], ls[RMAX], rc[RMAX], rs[RMAX];
   __shared__ double xa[160], xb[160];
+  __shared__ double red[2][2][128];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -149,32 +152,49 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
     double* M = Mw + (size_t)(ib - 1) * msz;
 
     // ---- deferred left rotations of the sweep's earlier columns:
-    //      A(:, col) and B(:, row0+r-1), local rows 1..L of block ib
+    //      A(:, col) and B(:, row0+r-1), local rows 1..L of block ib.
+    //      Thread t: output rows i = t%128 (+128), half h = t/128 of the
+    //      reduction; its loads of M are all in flight together.
     if (ic > 1 && L > 0) {
       const bool doB = row0 + r - 1 <= n;
       for (int t = tid; t < L; t += kGenThreads) {
         xa[t] = A[PHT_IDX(base + t, col - 1, lda)];
-        if (doB) xb[t] = B[PHT_IDX(base + t, row0 + r - 2, ldb)];
+        xb[t] = doB ? B[PHT_IDX(base + t, row0 + r - 2, ldb)] : 0.0;
       }
       __syncthreads();
-      for (int t = tid; t < 2 * L; t += kGenThreads) {
-        const bool isB = t >= L;
-        if (isB && !doB) continue;
-        const int i = isB ? t - L : t;
-        const double* x = isB ? xb : xa;
-        double acc0 = 0.0, acc1 = 0.0;
-        int j = 0;
-        for (; j + 1 < L; j += 2) {
-          acc0 += M[i + (size_t)j * ldr] * x[j];
-          acc1 += M[i + (size_t)(j + 1) * ldr] * x[j + 1];
+      const int h = tid >> 7, il = tid & 127;
+      const int jh = (L + 1) >> 1, j0 = h * jh, j1 = imin(L, j0 + jh);
+      for (int i0 = 0; i0 < L; i0 += 128) {
+        const int i = i0 + il;
+        double ya = 0.0, yb = 0.0;
+        if (i < L) {
+#pragma unroll
+          for (int q0 = 0; q0 < kMaxHalf; q0 += 16) {
+            double m[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+              const int j = j0 + q0 + q;
+              m[q] = (q0 + q < kMaxHalf && j < j1) ? M[i + (size_t)j * ldr] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+              const int j = j0 + q0 + q;
+              if (q0 + q < kMaxHalf && j < j1) {
+                ya += m[q] * xa[j];
+                yb += m[q] * xb[j];
+              }
+            }
+          }
         }
-        if (j < L) acc0 += M[i + (size_t)j * ldr] * x[j];
-        if (isB)
-          B[PHT_IDX(base + i, row0 + r - 2, ldb)] = acc0 + acc1;
-        else
-          A[PHT_IDX(base + i, col - 1, lda)] = acc0 + acc1;
+        red[0][h][il] = ya;
+        red[1][h][il] = yb;
+        __syncthreads();
+        if (h == 0 && i < L) {
+          A[PHT_IDX(base + i, col - 1, lda)] = red[0][0][il] + red[0][1][il];
+          if (doB) B[PHT_IDX(base + i, row0 + r - 2, ldb)] = red[1][0][il] + red[1][1][il];
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
 
     // the reference leaves the column here when nb < 2 (rht2ht.f90:827,911);
@@ -217,10 +237,21 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
         }
         if (lane < nb) A[PHT_IDX(R0 + lane, C0, lda)] = (lane == 0) ? g : 0.0;
       }
-      // B block into LDS (coalesced by rows)
-      for (int e = tid; e < nb * nb; e += kGenThreads) {
-        const int k = e % nb, j = e / nb;
-        Bb[k][j] = B[PHT_IDX(R0 + k, R0 + j, ldb)];
+      // B block into LDS (coalesced by rows; all loads in flight together)
+      {
+        constexpr int PER = (RMAX * RMAX + kGenThreads - 1) / kGenThreads;
+        double tmp[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int e = tid + q * kGenThreads;
+          const int k = e % RMAX, j = e / RMAX;
+          tmp[q] = (k < nb && j < nb) ? B[PHT_IDX(R0 + k, R0 + j, ldb)] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int e = tid + q * kGenThreads;
+          if (e < RMAX * RMAX) Bb[e % RMAX][e / RMAX] = tmp[q];
+        }
       }
       __syncthreads();
 
@@ -229,24 +260,47 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
       //      64.. fold the same rotations into M (local rows ic-1..ic+nb-2)
       if (tid < nb) {
         const int j = tid;
-        double carry = Bb[nb - 1][j];
-        for (int ig = nb - 1; ig >= 1; --ig) {
-          const double c = lc[ig], sn = ls[ig];
-          const double a = Bb[ig - 1][j];
-          Bb[ig][j] = c * carry - sn * a;
-          carry = c * a + sn * carry;
+        double w[RMAX];
+#pragma unroll
+        for (int k = 0; k < RMAX; ++k) w[k] = Bb[k][j];
+        double carry = 0.0;
+#pragma unroll
+        for (int ig = RMAX - 1; ig >= 1; --ig) {
+          if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
+          if (ig < nb) {
+            if (ig == nb - 1) carry = w[ig];
+            const double c = lc[ig], sn = ls[ig];
+            const double a = w[ig - 1];
+            w[ig] = c * carry - sn * a;
+            carry = c * a + sn * carry;
+          }
         }
-        Bb[0][j] = carry;
+        w[0] = (nb > 1) ? carry : w[0];
+#pragma unroll
+        for (int k = 0; k < RMAX; ++k)
+          if (k < nb) Bb[k][j] = w[k];
       } else if (tid >= 64 && tid < 64 + ldr) {
         double* p = M + (ic - 1) + (size_t)(tid - 64) * ldr;
-        double carry = p[nb - 1];
-        for (int ig = nb - 1; ig >= 1; --ig) {
-          const double c = lc[ig], sn = ls[ig];
-          const double a = p[ig - 1];
-          p[ig] = c * carry - sn * a;
-          carry = c * a + sn * carry;
+        const int lim = imin(nb, ldr - (ic - 1));  // rows of M this step touches
+        double w[RMAX];
+#pragma unroll
+        for (int k = 0; k < RMAX; ++k) w[k] = (k < lim) ? p[k] : 0.0;
+        double carry = 0.0;
+#pragma unroll
+        for (int ig = RMAX - 1; ig >= 1; --ig) {
+          if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
+          if (ig < nb) {
+            if (ig == nb - 1) carry = w[ig];
+            const double c = lc[ig], sn = ls[ig];
+            const double a = w[ig - 1];
+            w[ig] = c * carry - sn * a;
+            carry = c * a + sn * carry;
+          }
         }
-        p[0] = carry;
+        w[0] = (nb > 1) ? carry : w[0];
+#pragma unroll
+        for (int k = 0; k < RMAX; ++k)
+          if (k < lim) p[k] = w[k];
       }
       __syncthreads();
 
@@ -258,6 +312,7 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
         for (int j = 0; j < RMAX; ++j) brow[j] = (lane < nb && j < nb) ? Bb[lane][j] : 0.0;
 #pragma unroll
         for (int ig = RMAX - 1; ig >= 1; --ig) {
+          if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
           if (ig < nb) {
             const double f = __shfl(brow[ig], ig, 64);
             const double g = __shfl(brow[ig - 1], ig, 64);
@@ -319,6 +374,7 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
           double carry = 0.0;
 #pragma unroll
           for (int ig = RMAX - 1; ig >= 1; --ig) {
+          if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
             if (ig < nb) {
               if (ig == nb - 1) carry = seg[ig];
               const double c = rc[ig], sn = rs[ig];
@@ -451,86 +507,163 @@ __global__ __launch_bounds__(kTile) void k_s2_cols(
 }
 
 // ---------------------------------------------------------------------------
-// Register-window rotation sweeps for compile-time (r, nq).  A thread keeps a
-// window of R consecutive entries of its row (or column) in registers; the
-// rotations of column ic of a block act on window positions (ig-1, ig)
-// (local indices ic+ig-1, ic+ig), after which local index ic is final, is
-// stored, and the window slides by one.  Rotation coefficients are uniform
-// across the workgroup and come from the tape through scalar loads.
+// Register-window rotation sweeps for compile-time (r, nq).  A thread owns one
+// row (k_s2_rows_v) or column (k_s2_cols_v) of a matrix.  For every bulge
+// block it loads the block's whole window of R+NQ-1 entries at once (one
+// memory latency per block), and the block's NQ x (R-1) rotation
+// coefficients are staged in LDS by the workgroup.  The rotations of column
+// ic act on window positions (ic+ig-2, ic+ig-1); the window of R registers
+// slides by one position per ic (the NQ-1 entries beyond it wait in `ahead`),
+// so every register index is static.  Out-of-range rotations are identities
+// in the tape and out-of-range entries are zeros, so no bounds tests are
+// needed inside the rotation loop.
+namespace {
+// The NQ x (R-1) rotations of one bulge block, staged in LDS by the
+// workgroup: (c, s) of rotation (ic, ig) at cs[ic-1][ig] (ig >= 1).
 template <int R, int NQ>
-__global__ __launch_bounds__(256) void k_s2_rows_v(
+struct TapeBlock {
+  double2 cs[NQ][R];
+};
+
+// sgn = -1 stages the transposed rotations (left rotations seen from the
+// right, or the row form of a left rotation)
+template <int R, int NQ, int NT>
+__device__ __forceinline__ void stage_tape(TapeBlock<R, NQ>& tb, const double* __restrict__ Ct,
+                                           const double* __restrict__ St, int ib, double sgn) {
+  constexpr int LDR = R + NQ;
+  constexpr int PER = (NQ * R + NT - 1) / NT;
+  const size_t boff = (size_t)(ib - 1) * LDR * NQ;
+  // all loads first, then the LDS writes: one memory latency per block
+  double c[PER], sn[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * NT;
+    const int icm = e / R, ig = e % R;
+    const size_t ti = boff + (size_t)icm * LDR + (icm + ig - 1);
+    const bool ok = e < NQ * R && ig >= 1;
+    c[q] = ok ? Ct[ti] : 1.0;
+    sn[q] = ok ? St[ti] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * NT;
+    if (e < NQ * R) tb.cs[e / R][e % R] = make_double2(c[q], sgn * sn[q]);
+  }
+}
+
+// copies entries [0, W) of a window of NW entries between a row/column of
+// global memory (stride `stride` between entries) and an LDS column, in
+// groups of 16 entries whose loads are all in flight together
+template <int NW>
+__device__ __forceinline__ void window_to_lds(double (*tile)[65], int t, const double* src,
+                                              size_t stride, int W, bool ok) {
+#pragma unroll
+  for (int k0 = 0; k0 < NW; k0 += 16) {
+    double tmp[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      tmp[k] = (ok && k0 + k < W && k0 + k < NW) ? src[(k0 + k) * stride] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k0 + k < NW) tile[k0 + k][t] = tmp[k];
+  }
+}
+
+// applies column ic of the block to the sliding window v: rotation ig maps
+// (v[ig], v[ig-1]) to (c v[ig] + s v[ig-1], c v[ig-1] - s v[ig]).  The
+// coefficient reads are fenced in groups of 8 rotations so that at most a
+// group's worth of them is live.
+template <int R, int NQ>
+__device__ __forceinline__ void rotate_window(double (&v)[R], const TapeBlock<R, NQ>& tb,
+                                              int icm) {
+#pragma unroll
+  for (int ig = R - 1; ig >= 1; --ig) {
+    if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+    const double2 cs = tb.cs[icm][ig];
+    const double x = v[ig], y = v[ig - 1];
+    v[ig] = cs.x * x + cs.y * y;
+    v[ig - 1] = cs.x * y - cs.y * x;
+  }
+}
+
+// One block's window of one row, staged in the workgroup's LDS tile
+// (tile[k][t] = entry k of the window of the workgroup's row t).
+template <int R, int NQ, bool PARTIAL>
+__device__ __forceinline__ void rows_one_block(double (*tile)[65], int i, bool valid, int s,
+                                               int ib, int W, const TapeBlock<R, NQ>& tb) {
+  const int t = threadIdx.x;
+  double v[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = (k < W) ? tile[k][t] : 0.0;
+#pragma unroll 1
+  for (int icm = 0; icm < NQ; ++icm) {
+    bool act = valid;
+    if (PARTIAL) act = act && (i + 1 < imax(s + 1, s + 1 + (ib - NQ + icm) * R));
+    if (act) rotate_window<R, NQ>(v, tb, icm);
+    if (icm < W) tile[icm][t] = v[0];
+#pragma unroll
+    for (int k = 0; k < R - 1; ++k) v[k] = v[k + 1];
+    v[R - 1] = (R + icm < W) ? tile[R + icm][t] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < R - 1; ++k)
+    if (NQ + k < W) tile[NQ + k][t] = v[k];
+}
+}  // namespace
+
+constexpr int kRowsThreads = 64;
+
+// AB: rows of A (blockIdx.y = 0) and B (1) above the generation window, right
+// rotations; otherwise all rows of Q (0, left rotations seen from the right)
+// and Z (1, right rotations).
+template <int R, int NQ, bool AB>
+__global__ __launch_bounds__(kRowsThreads) void k_s2_rows_v(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb,
     double* __restrict__ Q, int ldq, double* __restrict__ Z, int ldz, int n,
     int s, int nblocks, const double* __restrict__ CU,
     const double* __restrict__ SU, const double* __restrict__ CV,
     const double* __restrict__ SV) {
-  constexpr int LDR = R + NQ;
-  const int which = blockIdx.y;
-  const int row_first = blockIdx.x * 256;
-  const int i = row_first + threadIdx.x;  // 0-based row
-  double* M;
-  int ld;
-  const double *Ct, *St;
-  switch (which) {
-    case 0: M = A; ld = lda; Ct = CV; St = SV; break;
-    case 1: M = B; ld = ldb; Ct = CV; St = SV; break;
-    case 2: M = Q; ld = ldq; Ct = CU; St = SU; break;
-    default: M = Z; ld = ldz; Ct = CV; St = SV; break;
-  }
-  const bool partial = which < 2;
+  constexpr int NW = R + NQ - 1;
+  __shared__ double tile[NW][65];
+  __shared__ TapeBlock<R, NQ> tb;
+  const int which = AB ? blockIdx.y : 2 + blockIdx.y;  // 0 A, 1 B, 2 Q, 3 Z
+  const int row_first = blockIdx.x * kRowsThreads;
+  const int t = threadIdx.x;
+  const int i = row_first + t;  // 0-based row
   const bool valid = i < n;
+  double* M = which == 0 ? A : which == 1 ? B : which == 2 ? Q : Z;
+  const int ld = which == 0 ? lda : which == 1 ? ldb : which == 2 ? ldq : ldz;
+  const double* Ct = which == 2 ? CU : CV;
+  const double* St = which == 2 ? SU : SV;
+  const double sgn = which == 2 ? -1.0 : 1.0;
   for (int ib = nblocks; ib >= 1; --ib) {
     const int cw0 = s + (ib - 1) * R;  // 0-based first window column
-    const int W = imin(LDR - 1, n - cw0);
+    const int W = imin(NW, n - cw0);
     if (W <= 0) continue;
-    if (partial && row_first + 1 >= imax(s + 1, s + 1 + (ib - 1) * R)) continue;
-    double v[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k)
-      v[k] = (valid && k < W) ? M[PHT_IDX(i, cw0 + k, ld)] : 0.0;
-    const size_t boff = (size_t)(ib - 1) * LDR * NQ;
-    for (int ic = 1; ic <= NQ; ++ic) {
-      bool act = valid;
-      if (partial) act = act && (i + 1 < imax(s + 1, s + 1 + (ib - 1 - NQ + ic) * R));
-      const double* cc = Ct + boff + (size_t)(ic - 1) * LDR + (ic - 2);
-      const double* ss = St + boff + (size_t)(ic - 1) * LDR + (ic - 2);
-      if (__any(act)) {
-#pragma unroll
-        for (int ig = R - 1; ig >= 1; --ig) {
-          if (ic + ig - 1 < W) {
-            const double c = cc[ig], sn = ss[ig];
-            const double x = v[ig], y = v[ig - 1];
-            double nx, ny;
-            if (which == 2) {
-              ny = c * y + sn * x;
-              nx = c * x - sn * y;
-            } else {
-              nx = c * x + sn * y;
-              ny = c * y - sn * x;
-            }
-            v[ig] = act ? nx : x;
-            v[ig - 1] = act ? ny : y;
-          }
-        }
-      }
-      if (valid && ic - 1 < W) M[PHT_IDX(i, cw0 + ic - 1, ld)] = v[0];
-#pragma unroll
-      for (int k = 0; k < R - 1; ++k) v[k] = v[k + 1];
-      v[R - 1] = (valid && ic - 1 + R < W) ? M[PHT_IDX(i, cw0 + ic - 1 + R, ld)] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < R - 1; ++k)
-      if (valid && NQ + k < W) M[PHT_IDX(i, cw0 + NQ + k, ld)] = v[k];
+    // A, B: rows at or below istart(nq, ib) are the generation's
+    if (AB && row_first + 1 >= imax(s + 1, s + 1 + (ib - 1) * R)) continue;
+    __syncthreads();  // the previous block's tile and coefficients are consumed
+    stage_tape<R, NQ, kRowsThreads>(tb, Ct, St, ib, sgn);
+    window_to_lds<NW>(tile, t, M + PHT_IDX(i, cw0, ld), (size_t)ld, W, valid);
+    __syncthreads();
+    rows_one_block<R, NQ, AB>(tile, i, valid, s, ib, W, tb);
+    if (valid)
+      for (int k = 0; k < W; ++k) M[PHT_IDX(i, cw0 + k, ld)] = tile[k][t];
   }
 }
 
+// Columns of A (which = 0) or B (1) right of the generation window: left
+// rotations on the rows of each block.  The 64 columns of a workgroup are
+// staged through LDS so that global accesses run along the columns.
 template <int R, int NQ>
 __global__ __launch_bounds__(64) void k_s2_cols_v(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
     int s, int nblocks, const double* __restrict__ CU,
     const double* __restrict__ SU) {
-  constexpr int LDR = R + NQ;
-  __shared__ double tile[LDR][65];
+  constexpr int NW = R + NQ - 1;
+  static_assert(NW <= 128, "two 64-row chunks per column");
+  __shared__ double tile[NW][65];
+  __shared__ TapeBlock<R, NQ> tb;
   const int which = blockIdx.y;
   const int t = threadIdx.x;
   const int j0 = blockIdx.x * 64;
@@ -542,35 +675,39 @@ __global__ __launch_bounds__(64) void k_s2_cols_v(
                                    : s + (ib - 1) * R + R + NQ - 1;  // 1-based
     if (j0 + 64 <= istop) continue;
     const int rw0 = s + (ib - 1) * R;  // 0-based first row
-    const int W = imin(LDR - 1, n - rw0);
+    const int W = imin(NW, n - rw0);
     if (W <= 0) continue;
     const int cmax = imin(64, n - j0);
-    for (int c = 0; c < cmax; ++c)
-      for (int k = t; k < W; k += 64) tile[k][c] = M[PHT_IDX(rw0 + k, j0 + c, ld)];
+    __syncthreads();  // the previous block's tile and coefficients are consumed
+    stage_tape<R, NQ, 64>(tb, CU, SU, ib, -1.0);
+    // 8 columns x 2 row chunks per group, all 16 loads in flight together
+#pragma unroll
+    for (int c0 = 0; c0 < 64; c0 += 8) {
+      double tmp[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int c = c0 + (q >> 1), k = t + 64 * (q & 1);
+        tmp[q] = (c < cmax && k < W) ? M[PHT_IDX(rw0 + k, j0 + c, ld)] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int c = c0 + (q >> 1), k = t + 64 * (q & 1);
+        if (k < NW) tile[k][c] = tmp[q];
+      }
+    }
     __syncthreads();
     const bool act = (j < n) && (j + 1 > istop);
     double v[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) v[k] = (k < W) ? tile[k][t] : 0.0;
-    const size_t boff = (size_t)(ib - 1) * LDR * NQ;
-    for (int ic = 1; ic <= NQ; ++ic) {
-      const double* cc = CU + boff + (size_t)(ic - 1) * LDR + (ic - 2);
-      const double* ss = SU + boff + (size_t)(ic - 1) * LDR + (ic - 2);
-      if (act) {
-#pragma unroll
-        for (int ig = R - 1; ig >= 1; --ig) {
-          if (ic + ig - 1 < W) {
-            const double c = cc[ig], sn = ss[ig];
-            const double a = v[ig - 1], b = v[ig];
-            v[ig - 1] = c * a + sn * b;
-            v[ig] = c * b - sn * a;
-          }
-        }
-      }
-      if (ic - 1 < W) tile[ic - 1][t] = v[0];
+#pragma unroll 1
+    for (int icm = 0; icm < NQ; ++icm) {
+      // the row form of a left rotation: the transposed coefficients
+      if (act) rotate_window<R, NQ>(v, tb, icm);
+      if (icm < W) tile[icm][t] = v[0];
 #pragma unroll
       for (int k = 0; k < R - 1; ++k) v[k] = v[k + 1];
-      v[R - 1] = (ic - 1 + R < W) ? tile[ic - 1 + R][t] : 0.0;
+      v[R - 1] = (R + icm < W) ? tile[R + icm][t] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < R - 1; ++k)
@@ -580,7 +717,6 @@ __global__ __launch_bounds__(64) void k_s2_cols_v(
       if (j0 + c + 1 <= istop) continue;
       for (int k = t; k < W; k += 64) M[PHT_IDX(rw0 + k, j0 + c, ld)] = tile[k][c];
     }
-    __syncthreads();
   }
 }
 
@@ -633,12 +769,16 @@ hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
     if (e != hipSuccess) return e;
     attr = true;
   }
-#define PHT_ROWS(RR, NN)                                                      \
-  if (P.r == RR && P.nq == NN) {                                              \
-    hipLaunchKernelGGL((k_s2_rows_v<RR, NN>), dim3((P.n + 255) / 256, 4),     \
-                       dim3(256), 0, st, A, lda, B, ldb, Q, ldq, Z, ldz, P.n, \
-                       P.jblock, P.nblocks, CU, SU, CV, SV);                  \
-    return hipGetLastError();                                                 \
+#define PHT_ROWS(RR, NN)                                                        \
+  if (P.r == RR && P.nq == NN) {                                                \
+    const dim3 g((P.n + kRowsThreads - 1) / kRowsThreads, 2);                   \
+    hipLaunchKernelGGL((k_s2_rows_v<RR, NN, true>), g, dim3(kRowsThreads), 0,   \
+                       st, A, lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,       \
+                       P.nblocks, CU, SU, CV, SV);                              \
+    hipLaunchKernelGGL((k_s2_rows_v<RR, NN, false>), g, dim3(kRowsThreads), 0,  \
+                       st, A, lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,       \
+                       P.nblocks, CU, SU, CV, SV);                              \
+    return hipGetLastError();                                                   \
   }
   PHT_S2_PAIRS(PHT_ROWS)
 #undef PHT_ROWS
