@@ -136,4 +136,34 @@ __device__ __forceinline__ void dlartg_d(double f, double g, double& c,
   }
 }
 
+// value of x in lane `src` of the wave (src wave-uniform, ideally a constant)
+__device__ __forceinline__ double readlane_d(double x, int src) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Givens rotation with dlartg's conventions (c >= 0, r with the sign of f,
+// g == 0 gives the identity) without division or square root: f and g are
+// scaled by the power of two that brings the larger magnitude into
+// [0.5, 1) (exact), 1/d comes from v_rsq_f64 (relative error below 2^-24)
+// refined by two Newton steps (below 2^-52), and c = |f|/d, s = g/r, r as
+// products.  c, s, r agree with dlartg to a few ulp for all finite inputs.
+__device__ __forceinline__ void rot_rsq(double f, double g, double& c, double& s, double& r) {
+  const double m = fmax(fabs(f), fabs(g));
+  const int k = (m > 0.0) ? __builtin_amdgcn_frexp_exp(m) : 0;
+  const double fs = ldexp(f, -k), gs = ldexp(g, -k);
+  const double d2 = fma(fs, fs, gs * gs);  // in [0.25, 2] unless f = g = 0
+  double y = __builtin_amdgcn_rsq(d2);
+  double e = fma(-d2 * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-d2 * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  const double sg = (f < 0.0) ? -y : y;
+  c = (g == 0.0) ? 1.0 : fabs(fs) * y;
+  s = (g == 0.0) ? 0.0 : gs * sg;
+  r = (g == 0.0) ? f : ldexp(d2 * sg, k);
+}
+
 }  // namespace pht

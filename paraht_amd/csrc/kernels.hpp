@@ -58,9 +58,17 @@ struct S2Params {
   int nblocks;           // bulge blocks of this sweep
   int ldr;               // r + nq
   double* M;             // nblocks x ldr x ldr workspace (deferred rotations)
-  int* progress;         // nq step counters of the sweep's workgroups
+  int* progress;         // nq step counters of the sweep's leader workgroups
+  int* hprog;            // nq x kS2MaxHelpers step counters of the helper workgroups
+  int nhelp;             // helper workgroups per column, 0..kS2MaxHelpers
   int* errflag;          // set when a workgroup's wait hit the spin limit
 };
+constexpr int kS2MaxHelpers = 4;
+// ints of the sweep's counters: progress, hprog (contiguous) and errflag
+inline size_t s2_counter_ints(int nq) { return (size_t)nq * (1 + kS2MaxHelpers) + 64; }
+// Dynamic LDS bytes of the sweep kernel for (r, nq), 0 if (r, nq) does not
+// fit (r <= 64 and M of (r+nq)^2 doubles in LDS beside the B block).
+size_t s2_sweep_lds(int r, int nq);
 // Resets the sweep's tape to identity rotations, M to identity and the
 // progress counters; must precede s2_generate.
 hipError_t s2_prepare(hipStream_t st, const S2Params& P, double* CU, double* SU,

@@ -31,6 +31,7 @@ struct KernelRec {
 
 struct paraht_ctx {
   int device = 0;
+  int cu_count = 256;  // compute units of the device
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   void* ws = nullptr;
@@ -388,17 +389,28 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
 // ------------------------------------------------------------------------
 // Stage 2 (Givens)
 // ------------------------------------------------------------------------
+// helper workgroups per stage-2 sweep column (PARAHT_S2_HELPERS overrides, 0..4);
+// they take the window rows above each bulge off the chase's critical path
+int s2_helpers() {
+  static int h = [] {
+    const char* e = getenv("PARAHT_S2_HELPERS");
+    int v = e ? atoi(e) : 2;
+    return v < 0 ? 0 : (v > pht::kS2MaxHelpers ? pht::kS2MaxHelpers : v);
+  }();
+  return h;
+}
+
 int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
            double* Q, int ldq, double* Z, int ldz, int r, int nq, int flags) {
   if (n <= 2) return 0;  // rht2ht.f90:1417 loop is empty
   if (r > 64) return PARAHT_ERR_UNSUPPORTED;
   hipStream_t st = c->stream;
   const int ldr = r + nq;
-  if (ldr > 160) return PARAHT_ERR_UNSUPPORTED;  // LDS windows of the sweeps
+  if (pht::s2_sweep_lds(r, nq) == 0) return PARAHT_ERR_UNSUPPORTED;  // M of a block in LDS
   int nbl = (n - 1) / r + ((n - 1) % r != 0);
   const size_t tsz = (size_t)ldr * nq * nbl;
   const size_t msz = (size_t)ldr * ldr * nbl;
-  if (int e = ensure_ws(c, (4 * tsz + msz) * sizeof(double) + (nq + 64) * sizeof(int)))
+  if (int e = ensure_ws(c, (4 * tsz + msz) * sizeof(double) + pht::s2_counter_ints(nq) * sizeof(int)))
     return e;
   double* CU = (double*)c->ws;
   double* SU = CU + tsz;
@@ -417,7 +429,12 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     P.ldr = ldr;
     P.M = SV + tsz;
     P.progress = (int*)(P.M + msz);
-    P.errflag = P.progress + nq;
+    P.hprog = P.progress + nq;
+    // helpers need every workgroup resident and nq <= r+1 (a column's steps
+    // then never read the rows above its earlier bulges)
+    P.nhelp = (nq <= r + 1) ? std::min(s2_helpers(), c->cu_count / nq - 1) : 0;
+    if (P.nhelp < 0) P.nhelp = 0;
+    P.errflag = P.progress + (size_t)nq * (1 + pht::kS2MaxHelpers);
     const int m0 = tm.mark();
     PROF(PARAHT_K_OTHER, 0.0, 0.0, pht::s2_prepare(st, P, CU, SU, CV, SV));
     PROF(PARAHT_K_S2_GEN, s2_gen_flops(n, r, nq, s), 0.0,
@@ -436,7 +453,8 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   }
   {
     int err = 0;
-    CK(hipMemcpyAsync(&err, (int*)(SV + tsz + msz) + nq, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&err, (int*)(SV + tsz + msz) + (size_t)nq * (1 + pht::kS2MaxHelpers),
+                       sizeof(int), hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     if (err) {
       if (verbose()) fprintf(stderr, "paraht: stage-2 sweep wait exceeded its spin limit\n");
@@ -574,6 +592,7 @@ int paraht_ctx_create(paraht_ctx** ctx, int device) {
   CK(hipSetDevice(device));
   paraht_ctx* c = new paraht_ctx();
   c->device = device;
+  c->cu_count = prop.multiProcessorCount;
   hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;

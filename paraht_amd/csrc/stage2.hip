@@ -30,7 +30,7 @@
 namespace pht {
 
 namespace {
-constexpr int kGenThreads = 256;
+constexpr int kGenThreads = 512;
 constexpr int kTile = 128;
 
 __device__ __forceinline__ size_t tape_idx(int loc1, int ic1, int ib1, int ldr,
@@ -61,9 +61,11 @@ __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 // reference's lazily-applied left rotations of a column (rht2ht.f90:803-825,
 // 887-909) are applied as one mat-vec with it.
 constexpr int kSpinLimit = 1 << 26;
+constexpr int kMaxHelpers = kS2MaxHelpers;
 constexpr int kDone = 1 << 29;
 constexpr int kMaxLdr = 160;                 // r + nq (host checks)
-constexpr int kMaxHalf = (kMaxLdr + 1) / 2;  // reduction half of the mat-vec
+constexpr int kParts = kGenThreads / 128;    // reduction parts of the mat-vec
+constexpr int kMaxPart = (kMaxLdr + kParts - 1) / kParts;
 
 __global__ void k_s2_prep(double* CU, double* SU, double* CV, double* SV,
                           size_t tsz, double* Mw, int ldr, int nblocks,
@@ -81,7 +83,7 @@ __global__ void k_s2_prep(double* CU, double* SU, double* CV, double* SV,
     const size_t k = e % msz;
     Mw[e] = ((k % ldr) == (k / ldr)) ? 1.0 : 0.0;
   }
-  for (size_t e = gid; e < (size_t)nq; e += gsz) progress[e] = 0;
+  for (size_t e = gid; e < (size_t)nq * (1 + kMaxHelpers); e += gsz) progress[e] = 0;
   if (gid == 0) *errflag = 0;
 }
 
@@ -103,19 +105,167 @@ __device__ __forceinline__ void rot_from_norm(double f, double g, double d,
   }
 }
 
+// Right rotations (rc, rs)[ig], ig = nb-1..1, of one step applied to row rr0
+// (0-based) of Mx on the step's columns R0..R0+nb-1.
+template <int RMAX>
+__device__ __forceinline__ void rotate_row(double* __restrict__ Mx, int ld, int rr0, int R0,
+                                           int nb, const double* rc, const double* rs) {
+  double seg[RMAX];
+#pragma unroll
+  for (int j = 0; j < RMAX; ++j) seg[j] = (j < nb) ? Mx[PHT_IDX(rr0, R0 + j, ld)] : 0.0;
+  double carry = 0.0;
+#pragma unroll
+  for (int ig = RMAX - 1; ig >= 1; --ig) {
+    if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
+    if (ig < nb) {
+      if (ig == nb - 1) carry = seg[ig];
+      const double c = rc[ig], sn = rs[ig];
+      const double y = seg[ig - 1];
+      seg[ig] = c * carry + sn * y;
+      carry = c * y - sn * carry;
+    }
+  }
+  seg[0] = carry;
+#pragma unroll
+  for (int j = 0; j < RMAX; ++j)
+    if (j < nb) Mx[PHT_IDX(rr0, R0 + j, ld)] = seg[j];
+}
+
+// Right chain on the B block Bb (one wave, lane = block row): rotations
+// ig = nb-1..1 on columns (ig-1, ig) zero B(ig, ig-1), the pivot values being
+// read from lane ig.  Rotation ig only mixes columns ig-1 and ig, and column
+// ig-1 is untouched before it, so a lane carries one running value (`cur`,
+// its entry in column ig) and streams the original column ig-1 from LDS;
+// column ig is final after rotation ig.  (rc, rs)[ig] receive the rotations.
+template <int RMAX>
+__device__ __forceinline__ void right_chain(double (*Bb)[RMAX + 1], int nb, int lane,
+                                            double* rc, double* rs) {
+  const bool row = lane < nb;
+  double cur = row ? Bb[lane][nb - 1] : 0.0;
+  double myc = 1.0, mys = 0.0;  // the rotation pivoted on this lane
+#pragma unroll 4
+  for (int ig = nb - 1; ig >= 1; --ig) {
+    const double y = row ? Bb[lane][ig - 1] : 0.0;
+    const double f = readlane_d(cur, ig);
+    const double g = readlane_d(y, ig);
+    double c, sn, rr;
+    rot_rsq(f, g, c, sn, rr);
+    const double nx = c * cur + sn * y, ny = c * y - sn * cur;
+    const bool piv = lane == ig, above = lane < ig;
+    if (row) Bb[lane][ig] = piv ? rr : (above ? nx : cur);
+    cur = piv ? 0.0 : (above ? ny : y);
+    myc = piv ? c : myc;
+    mys = piv ? sn : mys;
+  }
+  if (row) Bb[lane][0] = cur;
+  if (lane >= 1 && lane < nb) {
+    rc[lane] = myc;
+    rs[lane] = mys;
+  }
+}
+
+// Left rotations (lc, ls)[ig], ig = nb-1..1, on the entries p[k*st], k < nb,
+// of one LDS column: (a, b) = (p[ig-1], p[ig]) <- (c a + s b, c b - s a).
+// The reads do not depend on the carried value and run ahead of it.
+__device__ __forceinline__ void left_chain_lds(double* p, int st, int nb, const double* lc,
+                                               const double* ls) {
+  double carry = p[(nb - 1) * st];
+#pragma unroll 8
+  for (int ig = nb - 1; ig >= 1; --ig) {
+    const double c = lc[ig], sn = ls[ig];
+    const double a = p[(ig - 1) * st];
+    p[ig * st] = c * carry - sn * a;
+    carry = c * a + sn * carry;
+  }
+  p[0] = carry;
+}
+
+// Thread 0 spins (bounded) until *p >= target; the workgroup then acquires.
+__device__ __forceinline__ void wait_at_least(int* p, int target, int* errflag) {
+  if (threadIdx.x == 0) {
+    int spins = 0;
+    while (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(errflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+  }
+}
+
+// Helper workgroup h of column ic: after its column's step ib is published,
+// applies the step's right rotations to the window rows above the bulge,
+// rows istart..row0-1 of A and of B (rht2ht.f90:853-867,937-951), and
+// publishes hprog.  Its column's next steps never read these rows (nq <= r+1),
+// the next column's steps wait for them.
+template <int RMAX>
+__device__ void sweep_helper(double* __restrict__ A, int lda, double* __restrict__ B, int ldb,
+                             int n, int r, int nq, int s, const double* __restrict__ CV,
+                             const double* __restrict__ SV, int ldr, int* progress, int* hprog,
+                             int* errflag, int ic, int h, int nhelp, double* hc, double* hs) {
+  const int tid = threadIdx.x;
+  const int jc = s + ic - 1;
+  int* mine = &hprog[(ic - 1) * nhelp + h];
+  for (int ib = 1;; ++ib) {
+    int row0;
+    if (ib == 1) {
+      if (jc > n) break;
+      row0 = jc + 1;
+    } else {
+      const int col = jc + 1 + (ib - 2) * r;
+      if (col > n) break;
+      row0 = col + r;
+    }
+    wait_at_least(&progress[ic - 1], ib, errflag);
+    __syncthreads();
+    __threadfence();
+    const int nb = imin(r, n - row0 + 1);
+    const int istart = imax(s + 1, s + 1 + (ib - 1 - nq + ic) * r);
+    const int nrows = row0 - istart;  // rows istart..row0-1 (1-based)
+    if (nb >= 2 && nrows > 0) {
+      if (tid >= 1 && tid < nb) {
+        const size_t ti = tape_idx(ic + tid - 1, ic, ib, ldr, nq);
+        hc[tid] = CV[ti];
+        hs[tid] = SV[ti];
+      }
+      __syncthreads();
+      for (int t = h * kGenThreads + tid; t < 2 * nrows; t += nhelp * kGenThreads) {
+        const bool inA = t < nrows;
+        rotate_row<RMAX>(inA ? A : B, inA ? lda : ldb, istart - 1 + (inA ? t : t - nrows),
+                         row0 - 1, nb, hc, hs);
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(mine, ib, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(mine, kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int RMAX>
 __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
     int r, int nq, int s, double* __restrict__ CU, double* __restrict__ SU,
     double* __restrict__ CV, double* __restrict__ SV, int ldr,
-    double* __restrict__ Mw, int* progress, int* errflag) {
+    double* __restrict__ Mw, int* progress, int* errflag, int nhelp, int* hprog) {
   __shared__ double Bb[RMAX][RMAX + 1];
   __shared__ double lc[RMAX], ls[RMAX], rc[RMAX], rs[RMAX];
-  __shared__ double xa[160], xb[160];
-  __shared__ double red[2][2][128];
+  __shared__ double xa[kMaxLdr], xb[kMaxLdr];
+  __shared__ double red[2][kParts][128];
+  extern __shared__ double Ml[];  // M of the current block: M(i, j) at Ml[j*(ldr+1) + i]
+  const int ldl = ldr + 1;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
+  if ((int)blockIdx.x >= nq) {  // helper workgroups follow the nq leaders
+    const int hidx = blockIdx.x - nq;
+    sweep_helper<RMAX>(A, lda, B, ldb, n, r, nq, s, CV, SV, ldr, progress, hprog, errflag,
+                       hidx / nhelp + 1, hidx % nhelp, nhelp, rc, rs);
+    return;
+  }
   const int ic = blockIdx.x + 1;
   const int jc = s + ic - 1;
   const size_t msz = (size_t)ldr * ldr;
@@ -131,18 +281,9 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
       if (col > n) break;
       row0 = col + r;
     }
-    if (ic > 1) {
-      if (tid == 0) {
-        int spins = 0;
-        while (__hip_atomic_load(&progress[ic - 2], __ATOMIC_ACQUIRE,
-                                 __HIP_MEMORY_SCOPE_AGENT) < ib + 2) {
-          __builtin_amdgcn_s_sleep(2);
-          if (++spins > kSpinLimit) {
-            __hip_atomic_store(errflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-        }
-      }
+    if (ic > 1) {  // the previous column's leader and helpers are 2 steps ahead
+      wait_at_least(&progress[ic - 2], ib + 2, errflag);
+      for (int h = 0; h < nhelp; ++h) wait_at_least(&hprog[(ic - 2) * nhelp + h], ib + 2, errflag);
       __syncthreads();
       __threadfence();
     }
@@ -150,6 +291,22 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
     const int base = s + (ib - 1) * r;  // local row l of block ib = row base+l
     const int L = imin(ldr - 1, n - base);
     double* M = Mw + (size_t)(ib - 1) * msz;
+
+    // ---- M of this block into LDS (identity for the first column)
+    for (int e0 = 0; e0 < ldr * ldr; e0 += 16 * kGenThreads) {
+      double tmp[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int e = e0 + tid + q * kGenThreads;
+        tmp[q] = (ic > 1 && e < ldr * ldr) ? M[e] : (e % ldr == e / ldr ? 1.0 : 0.0);
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int e = e0 + tid + q * kGenThreads;
+        if (e < ldr * ldr) Ml[(e / ldr) * ldl + e % ldr] = tmp[q];
+      }
+    }
+    __syncthreads();
 
     // ---- deferred left rotations of the sweep's earlier columns:
     //      A(:, col) and B(:, row0+r-1), local rows 1..L of block ib.
@@ -163,35 +320,29 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
       }
       __syncthreads();
       const int h = tid >> 7, il = tid & 127;
-      const int jh = (L + 1) >> 1, j0 = h * jh, j1 = imin(L, j0 + jh);
+      const int jp = (L + kParts - 1) / kParts, j0 = h * jp, j1 = imin(L, j0 + jp);
       for (int i0 = 0; i0 < L; i0 += 128) {
         const int i = i0 + il;
         double ya = 0.0, yb = 0.0;
         if (i < L) {
-#pragma unroll
-          for (int q0 = 0; q0 < kMaxHalf; q0 += 16) {
-            double m[16];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-              const int j = j0 + q0 + q;
-              m[q] = (q0 + q < kMaxHalf && j < j1) ? M[i + (size_t)j * ldr] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-              const int j = j0 + q0 + q;
-              if (q0 + q < kMaxHalf && j < j1) {
-                ya += m[q] * xa[j];
-                yb += m[q] * xb[j];
-              }
-            }
+          for (int j = j0; j < j1; ++j) {
+            const double m = Ml[j * ldl + i];
+            ya += m * xa[j];
+            yb += m * xb[j];
           }
         }
         red[0][h][il] = ya;
         red[1][h][il] = yb;
         __syncthreads();
         if (h == 0 && i < L) {
-          A[PHT_IDX(base + i, col - 1, lda)] = red[0][0][il] + red[0][1][il];
-          if (doB) B[PHT_IDX(base + i, row0 + r - 2, ldb)] = red[1][0][il] + red[1][1][il];
+          double sa = 0.0, sb = 0.0;
+#pragma unroll
+          for (int q = 0; q < kParts; ++q) {
+            sa += red[0][q][il];
+            sb += red[1][q][il];
+          }
+          A[PHT_IDX(base + i, col - 1, lda)] = sa;
+          if (doB) B[PHT_IDX(base + i, row0 + r - 2, ldb)] = sb;
         }
         __syncthreads();
       }
@@ -258,86 +409,25 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
       // ---- left rotations on the B block (rht2ht.f90:837-840,921-924):
       //      threads 0..nb-1, one block column each; meanwhile threads
       //      64.. fold the same rotations into M (local rows ic-1..ic+nb-2)
-      if (tid < nb) {
-        const int j = tid;
-        double w[RMAX];
-#pragma unroll
-        for (int k = 0; k < RMAX; ++k) w[k] = Bb[k][j];
-        double carry = 0.0;
-#pragma unroll
-        for (int ig = RMAX - 1; ig >= 1; --ig) {
-          if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
-          if (ig < nb) {
-            if (ig == nb - 1) carry = w[ig];
-            const double c = lc[ig], sn = ls[ig];
-            const double a = w[ig - 1];
-            w[ig] = c * carry - sn * a;
-            carry = c * a + sn * carry;
-          }
-        }
-        w[0] = (nb > 1) ? carry : w[0];
-#pragma unroll
-        for (int k = 0; k < RMAX; ++k)
-          if (k < nb) Bb[k][j] = w[k];
-      } else if (tid >= 64 && tid < 64 + ldr) {
-        double* p = M + (ic - 1) + (size_t)(tid - 64) * ldr;
-        const int lim = imin(nb, ldr - (ic - 1));  // rows of M this step touches
-        double w[RMAX];
-#pragma unroll
-        for (int k = 0; k < RMAX; ++k) w[k] = (k < lim) ? p[k] : 0.0;
-        double carry = 0.0;
-#pragma unroll
-        for (int ig = RMAX - 1; ig >= 1; --ig) {
-          if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
-          if (ig < nb) {
-            if (ig == nb - 1) carry = w[ig];
-            const double c = lc[ig], sn = ls[ig];
-            const double a = w[ig - 1];
-            w[ig] = c * carry - sn * a;
-            carry = c * a + sn * carry;
-          }
-        }
-        w[0] = (nb > 1) ? carry : w[0];
-#pragma unroll
-        for (int k = 0; k < RMAX; ++k)
-          if (k < lim) p[k] = w[k];
-      }
+      if (tid < nb)
+        left_chain_lds(&Bb[0][tid], RMAX + 1, nb, lc, ls);
+      else if (tid >= 64 && tid < 64 + ldr)
+        left_chain_lds(&Ml[(tid - 64) * ldl + (ic - 1)], 1, nb, lc, ls);
       __syncthreads();
+      // the updated M rows go back while wave 0 runs the right chain
+      if (wave > 0) {
+        constexpr int PM = (RMAX * kMaxLdr + kGenThreads - 64 - 1) / (kGenThreads - 64);
+#pragma unroll
+        for (int q = 0; q < PM; ++q) {
+          const int e = (tid - 64) + q * (kGenThreads - 64);
+          const int k = e % RMAX, cm = e / RMAX;
+          if (k < nb && cm < ldr) M[(ic - 1) + k + (size_t)cm * ldr] = Ml[cm * ldl + (ic - 1) + k];
+        }
+      }
 
       // ---- right chain on the B block (rht2ht.f90:843-851,927-935):
-      //      wave 0, lane = block row, rotations broadcast by shuffles
-      if (wave == 0) {
-        double brow[RMAX];
-#pragma unroll
-        for (int j = 0; j < RMAX; ++j) brow[j] = (lane < nb && j < nb) ? Bb[lane][j] : 0.0;
-#pragma unroll
-        for (int ig = RMAX - 1; ig >= 1; --ig) {
-          if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
-          if (ig < nb) {
-            const double f = __shfl(brow[ig], ig, 64);
-            const double g = __shfl(brow[ig - 1], ig, 64);
-            double c, sn, rr;
-            dlartg_d(f, g, c, sn, rr);
-            if (lane == ig) {
-              brow[ig] = rr;
-              brow[ig - 1] = 0.0;
-            } else if (lane < ig) {
-              const double x = brow[ig], y = brow[ig - 1];
-              brow[ig] = c * x + sn * y;
-              brow[ig - 1] = c * y - sn * x;
-            }
-            if (lane == 0) {
-              rc[ig] = c;
-              rs[ig] = sn;
-            }
-          }
-        }
-        if (lane < nb) {
-#pragma unroll
-          for (int j = 0; j < RMAX; ++j)
-            if (j < nb) Bb[lane][j] = brow[j];
-        }
-      }
+      //      wave 0, lane = block row; the pivot values are read from lane ig
+      if (wave == 0) right_chain<RMAX>(Bb, nb, lane, rc, rs);
       __syncthreads();
       for (int e = tid; e < nb * nb; e += kGenThreads) {
         const int k = e % nb, j = e / nb;
@@ -349,44 +439,18 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
         SV[ti] = rs[tid];
       }
 
-      // ---- right rotations on the window rows: A rows istart..row0+r+nb-1,
-      //      B rows istart..row0-1 (rht2ht.f90:853-867,937-951)
+      // ---- right rotations on the window rows (rht2ht.f90:853-867,937-951):
+      //      A rows istart..row0+r+nb-1 and B rows istart..row0-1; with helper
+      //      workgroups the rows above the bulge (istart..row0-1) are theirs
       {
         const int aend = imin(n, row0 + r + nb - 1);  // 1-based inclusive
-        const int nAr = aend - istart + 1;
-        const int nBr = imax(0, row0 - istart);
+        const int first = (nhelp > 0) ? row0 : istart;
+        const int nAr = aend - first + 1;
+        const int nBr = (nhelp > 0) ? 0 : row0 - istart;
         for (int t = tid; t < nAr + nBr; t += kGenThreads) {
-          double* Mx;
-          int ld, rr0;
-          if (t < nAr) {
-            Mx = A;
-            ld = lda;
-            rr0 = istart - 1 + t;
-          } else {
-            Mx = B;
-            ld = ldb;
-            rr0 = istart - 1 + (t - nAr);
-          }
-          double seg[RMAX];
-#pragma unroll
-          for (int j = 0; j < RMAX; ++j)
-            seg[j] = (j < nb) ? Mx[PHT_IDX(rr0, R0 + j, ld)] : 0.0;
-          double carry = 0.0;
-#pragma unroll
-          for (int ig = RMAX - 1; ig >= 1; --ig) {
-          if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
-            if (ig < nb) {
-              if (ig == nb - 1) carry = seg[ig];
-              const double c = rc[ig], sn = rs[ig];
-              const double y = seg[ig - 1];
-              seg[ig] = c * carry + sn * y;
-              carry = c * y - sn * carry;
-            }
-          }
-          seg[0] = carry;
-#pragma unroll
-          for (int j = 0; j < RMAX; ++j)
-            if (j < nb) Mx[PHT_IDX(rr0, R0 + j, ld)] = seg[j];
+          const bool inA = t < nAr;
+          rotate_row<RMAX>(inA ? A : B, inA ? lda : ldb, first - 1 + (inA ? t : t - nAr), R0,
+                           nb, rc, rs);
         }
       }
     }
@@ -726,6 +790,22 @@ __global__ __launch_bounds__(64) void k_s2_cols_v(
   X(64, 8) X(64, 16) X(64, 32)
 
 // ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kLdsBudget = 160 * 1024;
+// static LDS of k_s2_sweep<R> (Bb, rotation vectors, x vectors, partial sums)
+constexpr size_t sweep_static_lds(int R) {
+  return sizeof(double) * ((size_t)R * (R + 1) + 4 * R + 2 * kMaxLdr + 2 * kParts * 128);
+}
+int sweep_rmax(int r) { return r <= 8 ? 8 : r <= 16 ? 16 : r <= 32 ? 32 : 64; }
+}  // namespace
+
+size_t s2_sweep_lds(int r, int nq) {
+  const size_t ldr = (size_t)r + nq;
+  if (r > 64 || ldr > (size_t)kMaxLdr) return 0;
+  const size_t dyn = ldr * (ldr + 1) * sizeof(double);
+  return (sweep_static_lds(sweep_rmax(r)) + dyn <= kLdsBudget) ? dyn : 0;
+}
+
 hipError_t s2_prepare(hipStream_t st, const S2Params& P, double* CU, double* SU,
                       double* CV, double* SV) {
   const size_t tsz = (size_t)P.ldr * P.nq * P.nblocks;
@@ -737,10 +817,23 @@ hipError_t s2_prepare(hipStream_t st, const S2Params& P, double* CU, double* SU,
 hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
                        const S2Params& P, double* CU, double* SU, double* CV,
                        double* SV) {
-  const dim3 g(P.nq), b(kGenThreads);
-#define PHT_GEN(R)                                                              \
-  hipLaunchKernelGGL(k_s2_sweep<R>, g, b, 0, st, A, lda, B, ldb, P.n, P.r, P.nq, \
-                     P.jblock, CU, SU, CV, SV, P.ldr, P.M, P.progress, P.errflag)
+  const dim3 g(P.nq * (1 + P.nhelp)), b(kGenThreads);
+  const size_t lds = s2_sweep_lds(P.r, P.nq);
+  if (lds == 0) return hipErrorInvalidValue;
+#define PHT_GEN(R)                                                                \
+  do {                                                                            \
+    static bool attr = false;                                                     \
+    if (!attr) {                                                                  \
+      hipError_t e = hipFuncSetAttribute((const void*)k_s2_sweep<R>,              \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                         (int)(kLdsBudget - sweep_static_lds(R)));   \
+      if (e != hipSuccess) return e;                                              \
+      attr = true;                                                                \
+    }                                                                             \
+    hipLaunchKernelGGL(k_s2_sweep<R>, g, b, lds, st, A, lda, B, ldb, P.n, P.r, P.nq, \
+                       P.jblock, CU, SU, CV, SV, P.ldr, P.M, P.progress, P.errflag, \
+                       P.nhelp, P.hprog);                                          \
+  } while (0)
   if (P.r <= 8)
     PHT_GEN(8);
   else if (P.r <= 16)
