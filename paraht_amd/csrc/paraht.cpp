@@ -481,7 +481,8 @@ std::mutex g_mu;
 paraht_ctx* g_default[64] = {nullptr};
 
 int default_ctx(paraht_ctx** out) {
-  int dev = 0;
+  int dev = 0, ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PARAHT_ERR_NODEV;
   CK(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return PARAHT_ERR_NODEV;
   std::lock_guard<std::mutex> lk(g_mu);

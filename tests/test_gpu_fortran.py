@@ -1,0 +1,35 @@
+"""The Fortran drop-in driver on the GPU: algorithm 1 runs both stages through
+the ab2rht / rht2ht modules and the C-ABI, and reports backward errors within
+10 n u in the reference's result line (src/profilereduction.f90:176-185)."""
+import re
+import subprocess
+
+import pytest
+
+from paraht_amd import dropin
+
+pytestmark = pytest.mark.gpu
+
+LINE = re.compile(r"n:\s*(\d+) ;runtime:\s*([-\d.]+) ;error:\s*(\S+)\s+(\S+) ;phase 1 time:\s*([-\d.]+)"
+                  r" ;phase 2 time:\s*([-\d.]+) ;r:\s*(\d+) ;p:\s*(\d+) ;nq:\s*(\d+)")
+
+
+@pytest.mark.parametrize("n,r,p,nq", [(256, 16, 4, 8), (300, 8, 2, 4)])
+def test_driver_paraht(n, r, p, nq):
+    print(f"fortran driver n={n} r={r} p={p} nq={nq}", flush=True)
+    out = subprocess.run([dropin.DRIVER, str(n), "1", str(r), str(p), str(nq)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    m = LINE.search(out.stdout)
+    assert m, out.stdout
+    assert (int(m.group(1)), int(m.group(7)), int(m.group(8)), int(m.group(9))) == (n, r, p, nq)
+    u = 2.0 ** -53
+    assert float(m.group(3)) <= 10 * n * u and float(m.group(4)) <= 10 * n * u, out.stdout
+
+
+def test_driver_time_components():
+    out = subprocess.run([dropin.DRIVER, "200", "1", "16", "2", "8", "1"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "genlefttime" in out.stdout and "gen:" in out.stdout, out.stdout
+    assert LINE.search(out.stdout), out.stdout
