@@ -75,6 +75,72 @@ def cpu_baseline(args):
                       f"scaled by (n/{ns})^3 to n={args.n}"}
 
 
+def bench_batched(args):
+    """Batched path: 2048 FP64 pencils with n uniform in [32, 512] (A ~ N(0,1),
+    B = R of the QR of an N(0,1) matrix, Q = Z = I), reduced in one
+    HIP-graph-captured persistent launch with the size-bucket (r, p, nq);
+    pencils/s over 3 timed launches, against a loop of the single-pencil
+    path over the first `batched_loop_sample` pencils of the same draw."""
+    import numpy as np
+    import torch
+
+    import paraht_amd as ph
+    from paraht_amd.batched import Batch, defaults, reduce_batch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1302)
+    sizes = [int(x) for x in rng.integers(32, 513, size=2048)]
+    g = torch.Generator(device=dev)
+    g.manual_seed(1302)
+    As, Bs = [], []
+    for n in sizes:
+        As.append(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+        R = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g),
+                            mode="r")[1]
+        Bs.append(torch.triu(R))
+    bt = Batch(sizes)
+    desc = bt.descriptors()
+    bt.load(As, Bs)
+    reduce_batch(bt, desc=desc, graph=True)  # warmup (and graph capture)
+    ts = []
+    for _ in range(3):
+        bt.load(As, Bs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reduce_batch(bt, desc=desc, graph=True)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_b = sum(ts) / len(ts)
+    worst = 0.0
+    for i in range(0, len(sizes), 64):  # backward errors of every 64th pencil
+        H, T, Q, Z = (bt.view(M, i) for M in (bt.A, bt.B, bt.Q, bt.Z))
+        ea = (torch.linalg.norm(Q.T @ As[i] @ Z - H) / torch.linalg.norm(As[i])).item()
+        eb = (torch.linalg.norm(Q.T @ Bs[i] @ Z - T) / torch.linalg.norm(Bs[i])).item()
+        worst = max(worst, ea, eb)
+    k = min(args.batched_loop_sample, len(sizes))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(k):
+        r, p, nq = defaults(sizes[i])
+        ph.reduce(ph.column_major(As[i].clone()), ph.column_major(Bs[i].clone()), r, p, nq)
+    torch.cuda.synchronize()
+    t_l = time.perf_counter() - t0
+    # the loop's rate scaled by the sample's share of the 14 n^3 work
+    work_share = sum(n ** 3 for n in sizes[:k]) / (sum(n ** 3 for n in sizes) / len(sizes) * k)
+    loop_rate = k / t_l * work_share
+    rate = len(sizes) / t_b
+    print(f"batched: {len(sizes)} pencils in {t_b:.3f} s; loop {k} pencils in {t_l:.3f} s",
+          file=sys.stderr, flush=True)
+    return {"metric": "batched pencils/s", "pencils": len(sizes), "n_range": [32, 512],
+            "s": t_b, "pencils_per_s": rate, "gflops": sum(14.0 * n ** 3 for n in sizes) / t_b / 1e9,
+            "params": "size-bucket defaults " + str(sorted({defaults(n) for n in sizes})),
+            "launch": "one HIP-graph-captured persistent launch",
+            "loop_baseline": {"pencils": k, "s": t_l, "pencils_per_s": loop_rate,
+                              "note": "single-pencil path per pencil, rate scaled to the "
+                                      "batch's mean n^3"},
+            "speedup_vs_loop": rate / loop_rate,
+            "max_backward_error_every_64th": worst}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +153,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=1536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched section (2048 pencils, n uniform in [32, 512])")
+    ap.add_argument("--batched-loop-sample", type=int, default=48,
+                    help="pencils timed through the single-pencil path for the loop baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="before the warmup, launch a plain elementwise copy of "
                          "1 GiB three times (the reference kernel for scaling "
@@ -200,6 +270,8 @@ def main():
                                "traffic": traffic_per_launch(dom, args),
                                "avg_launch_us": t_launch * 1e6, "launches": cnt // args.steps,
                                "algorithmic_per_launch": (fl if bound == "mfma" else by) / cnt}
+    if not args.no_batched:
+        out["batched"] = bench_batched(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

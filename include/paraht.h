@@ -111,6 +111,29 @@ int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
                       int ldb, double* dQ, int ldq, double* dZ, int ldz, int r,
                       int p, int nq, int flags);
 
+/* ---- Batched reduction of many small pencils ----
+ * No counterpart in the reference.  Reduces `count` independent pencils with
+ * both stages in one persistent launch: one workgroup per compute unit takes
+ * pencils from a work queue ordered by decreasing n^3 and reduces each by
+ * itself.  pencils[i].offset is the element offset of pencil i's n x n
+ * column-major matrices (leading dimension n) in each of the device pools
+ * dA, dB, dQ, dZ; the matrices are reduced in place (Q, Z accumulated).
+ * r, p, nq of 0 take the library's size-bucket defaults
+ * (paraht_batched_defaults); supported: r <= 16, p*r <= 128,
+ * (r, nq) in {(8,4), (8,8), (16,4), (16,8), (16,16)}.
+ * PARAHT_BATCH_GRAPH captures the launch in a HIP graph and replays it while
+ * the arguments repeat. */
+typedef struct {
+  long long offset;
+  int n, r, p, nq;
+} paraht_pencil;
+#define PARAHT_BATCH_GRAPH 0x4
+int paraht_batched_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils, double* dA,
+                       double* dB, double* dQ, double* dZ, int flags);
+/* The (r, p, nq) the batched path uses for a pencil of order n when the
+ * caller passes zeros (the size-bucket table). */
+int paraht_batched_defaults(int n, int* r, int* p, int* nq);
+
 /* Library version (major*10000 + minor*100 + patch). */
 int paraht_version(void);
 /* Human-readable text of a status code. */

@@ -31,7 +31,7 @@ from ._lib import PARAHT_STAGE2_GIVENS, PARAHT_TIME_COMPONENTS, ParahtError
 __all__ = [
     "ab2rht_simple", "rht2ht_householder", "rht2ht_givens",
     "rht2ht_householder_unblocked", "reduce", "Context", "ParahtError",
-    "build", "column_major",
+    "build", "column_major", "batched",
 ]
 
 build = _lib.build
@@ -206,3 +206,6 @@ def reduce(A, B, r, p, nq, Q=None, Z=None, time_components=False, ctx=None):
     _dispatch("paraht_reduce", None, "reduce", n,
               [(H, n), (T, n), (Q, n), (Z, n)], [r, p, nq], flags)
     return H, T, Q, Z
+
+
+from . import batched  # noqa: E402  (batched reduction of many small pencils)

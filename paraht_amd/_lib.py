@@ -15,6 +15,13 @@ _lib = None
 
 PARAHT_TIME_COMPONENTS = 0x1
 PARAHT_STAGE2_GIVENS = 0x2
+PARAHT_BATCH_GRAPH = 0x4
+
+
+class Pencil(ctypes.Structure):
+    """paraht_pencil of include/paraht.h: one pencil of a batch."""
+    _fields_ = [("offset", ctypes.c_longlong), ("n", ctypes.c_int), ("r", ctypes.c_int),
+                ("p", ctypes.c_int), ("nq", ctypes.c_int)]
 
 # (name, argtypes) of every entry point of include/paraht.h
 _P = ctypes.c_void_p
@@ -38,6 +45,9 @@ SIGNATURES = {
     "paraht_ab2rht_dev": ([_P] + _SIG_HOST12, _I),
     "paraht_rht2ht_dev": ([_P] + _SIG_HOST12, _I),
     "paraht_reduce_dev": ([_P] + _SIG_HOST12 + [_I], _I),
+    "paraht_batched_dev": ([_P, _I, ctypes.POINTER(Pencil), _DP, _DP, _DP, _DP, _I], _I),
+    "paraht_batched_defaults": ([_I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                 ctypes.POINTER(_I)], _I),
     "paraht_version": ([], _I),
     "paraht_status_string": ([_I], ctypes.c_char_p),
 }

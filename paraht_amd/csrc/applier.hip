@@ -18,15 +18,9 @@
 // 4*i + lane/16 of its tile, so the i-th MFMA of a 16-deep chunk takes its
 // reduction indices 4*i + slot from register i directly: the reduction index
 // is permuted within each chunk and no data moves across lanes.
-#include "device_util.hpp"
-#include "kernels.hpp"
+#include "apply_dev.hpp"
 
 namespace pht {
-
-template <int SIDE>
-__device__ __forceinline__ size_t cidx(int span, int other, int ldc) {
-  return SIDE == kLeftT ? PHT_IDX(span, other, ldc) : PHT_IDX(other, span, ldc);
-}
 
 template <int SIDE, int NKT>
 __global__ __launch_bounds__(256) void k_apply_blocks(
@@ -35,86 +29,12 @@ __global__ __launch_bounds__(256) void k_apply_blocks(
     int nblocks, int other_begin) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int l16 = lane & 15;
-  const int q = lane >> 4;
   const int o0 = other_begin + (blockIdx.x * 4 + wave) * 16;
-  const int oi = o0 + l16;
 
   for (int b = 0; b < nblocks; ++b) {
     const ApplyBlock bk = blocks[b];
-    const bool active = (o0 < bk.hi) && (o0 + 16 > bk.lo);
-    if (active) {
-      const bool omask = (oi >= bk.lo) && (oi < bk.hi);
-      const double* Vb = V + bk.voff;
-      const double* Tb = T + bk.toff;
-      const int len = bk.len, k = bk.k, st = bk.start;
-      const int nct = (len + 15) >> 4;
-
-      dbl4 W[NKT];
-#pragma unroll
-      for (int t = 0; t < NKT; ++t) W[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-
-      // phase 1: W = V^T C_slice
-      for (int ct = 0; ct < nct; ++ct) {
-        double c[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int s = 16 * ct + 4 * i + q;
-          c[i] = (omask && s < len) ? C[cidx<SIDE>(st + s, oi, ldc)] : 0.0;
-        }
-#pragma unroll
-        for (int kt = 0; kt < NKT; ++kt) {
-          const int m = 16 * kt + l16;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int s = 16 * ct + 4 * i + q;
-            const double a = (s < len && m < k) ? Vb[PHT_IDX(s, m, ldv)] : 0.0;
-            W[kt] = mfma16(a, c[i], W[kt]);
-          }
-        }
-      }
-      // phase 2: W2 = T^T W
-      dbl4 W2[NKT];
-#pragma unroll
-      for (int mt = 0; mt < NKT; ++mt) {
-        W2[mt] = dbl4{0.0, 0.0, 0.0, 0.0};
-        const int m = 16 * mt + l16;
-#pragma unroll
-        for (int kt = 0; kt < NKT; ++kt) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int kk = 16 * kt + 4 * i + q;
-            const double a = (kk < k && m < k) ? Tb[PHT_IDX(kk, m, ldt)] : 0.0;
-            W2[mt] = mfma16(a, W[kt][i], W2[mt]);
-          }
-        }
-      }
-      // phase 3: C_slice -= V W2
-      for (int ct = 0; ct < nct; ++ct) {
-        dbl4 acc;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int s = 16 * ct + 4 * i + q;
-          acc[i] = (omask && s < len) ? C[cidx<SIDE>(st + s, oi, ldc)] : 0.0;
-        }
-        const int srow = 16 * ct + l16;
-#pragma unroll
-        for (int kt = 0; kt < NKT; ++kt) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int m = 16 * kt + 4 * i + q;
-            const double a =
-                (srow < len && m < k) ? -Vb[PHT_IDX(srow, m, ldv)] : 0.0;
-            acc = mfma16(a, W2[kt][i], acc);
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int s = 16 * ct + 4 * i + q;
-          if (omask && s < len) C[cidx<SIDE>(st + s, oi, ldc)] = acc[i];
-        }
-      }
-    }
+    if ((o0 < bk.hi) && (o0 + 16 > bk.lo))
+      apply_block_slice<SIDE, NKT>(C, ldc, V, ldv, T, ldt, bk, o0, lane);
     // consecutive blocks overlap in the span dimension: make this wave's
     // stores visible before the next block's loads
     __syncthreads();

@@ -1,0 +1,356 @@
+// Batched persistent reduction of many independent small pencils.
+//
+// One launch of k_batched reduces a whole batch.  There is one 1024-thread
+// workgroup per compute unit; each takes pencils from an atomic work queue
+// (the host orders the queue by decreasing n^3) and runs the complete
+// two-stage reduction of that pencil by itself, with the same algorithms as
+// the single-pencil path:
+//  * stage 1 (ab2rht_simple, src/ab2rht.f90:8-495): the panel QR and RQ
+//    generators of s1_dev.hpp and the compact-WY MFMA applier of
+//    apply_dev.hpp, its 16 waves splitting the other dimension of each update;
+//  * stage 2 (rht2ht_givens semantics, src/rht2ht.f90:696-1344): the chase
+//    steps of s2_dev.hpp in the reference's order (column by column, block by
+//    block, each with its whole window), then the far updates of the sweep
+//    with register windows.
+// Every pencil has its own (r, p, nq), normally from the size-bucket table
+// (batched_defaults) or the autotuner; r <= 16, p*r <= 128 and (r, nq) one of
+// the pairs of PHT_BATCH_PAIRS.
+#include "apply_dev.hpp"
+#include "s1_dev.hpp"
+#include "s2_dev.hpp"
+
+namespace pht {
+
+namespace {
+constexpr int kBT = 1024;      // threads of a batched workgroup (= s1::kT)
+constexpr int kBRmax = 16;     // largest r of the batched path
+constexpr int kBWaves = kBT / 64;
+static_assert(kBT == s1::kT, "the stage-1 bodies assume 1024 threads");
+
+#define PHT_BATCH_PAIRS(X) X(8, 4) X(8, 8) X(16, 4) X(16, 8) X(16, 16)
+
+// ---- stage 1 -------------------------------------------------------------
+
+// Block bl of the panel at 0-based column jb (bottom-up order, ab2rht.f90:94-99)
+struct PanelGeom {
+  int n, nb, p, jb, nblocks, P;
+  __device__ ApplyBlock block(int bl, int lo, int hi, bool right) const {
+    const int i0 = jb + nb + (nblocks - 1 - bl) * (p - 1) * nb;
+    const int len = min(n - i0, P);
+    const int k = right ? ((len > nb) ? nb : len) : min(nb, len);
+    ApplyBlock b;
+    b.start = i0;
+    b.len = len;
+    b.k = k;
+    b.lo = lo < 0 ? i0 : lo;
+    b.hi = hi < 0 ? i0 + len : hi;
+    b.pad_ = 0;
+    b.voff = (long long)bl * P * nb;
+    b.toff = (long long)bl * nb * nb;
+    return b;
+  }
+};
+
+// Blocks [b0, b1) of a panel applied to C (other dimension [olo, ohi)) by
+// the 16 waves of the workgroup, each owning 16-index slices; lo/hi of -1
+// take the block's own start / end.
+template <int SIDE>
+__device__ void apply_wg(double* C, int ldc, const double* V, int ldv, const double* T, int ldt,
+                         const PanelGeom& g, int b0, int b1, int lo, int hi, bool right,
+                         int olo, int ohi) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o0 = olo + 16 * wave; o0 < ohi; o0 += 16 * kBWaves) {
+    for (int bl = b0; bl < b1; ++bl) {
+      const ApplyBlock bk = g.block(bl, lo, hi, right);
+      if (o0 < bk.hi && o0 + 16 > bk.lo)
+        apply_block_slice<SIDE, 1>(C, ldc, V, ldv, T, ldt, bk, o0, lane);
+      __threadfence_block();  // the next block reads what this one wrote
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __noinline__ void stage1_wg(double* sm, double* A, double* B, double* Q, double* Z, int n, int nb,
+                          int p, double* ws) {
+  const int P = p * nb;
+  const int nbmax = (n - nb + (p - 1) * nb - 1) / ((p - 1) * nb) + 1;
+  double* VL = ws;
+  double* VR = VL + (size_t)nbmax * P * nb;
+  double* TL = VR + (size_t)nbmax * P * nb;
+  double* TR = TL + (size_t)nbmax * nb * nb;
+  for (int jb = 0; jb <= n - nb - 2; jb += nb) {
+    int nblocks = (n - nb - jb) / ((p - 1) * nb);
+    if ((n - nb - jb) % ((p - 1) * nb)) ++nblocks;
+    const PanelGeom g{n, nb, p, jb, nblocks, P};
+    // left generation and application (ab2rht.f90:85-250)
+    s1::panel_qr_body(sm, A, n, n, jb, nb, p, nblocks, VL, P, (long long)P * nb, TL, nb,
+                      (long long)nb * nb);
+    __syncthreads();
+    apply_wg<kLeftT>(B, n, VL, P, TL, nb, g, 0, nblocks, -1, n, false, jb + nb, n);
+    apply_wg<kLeftT>(A, n, VL, P, TL, nb, g, 0, nblocks, jb + nb, n, false, jb + nb, n);
+    apply_wg<kRightN>(Q, n, VL, P, TL, nb, g, 0, nblocks, 0, n, false, 0, n);
+    // right generation, block by block, with B's update (ab2rht.f90:270-336)
+    for (int bl = 0; bl < nblocks; ++bl) {
+      const ApplyBlock bk = g.block(bl, 0, -1, true);
+      s1::rq_gen_body(sm, B, n, bk.start, bk.len, nb, VR + bk.voff, P, TR + bk.toff, nb);
+      __threadfence_block();
+      __syncthreads();
+      apply_wg<kRightN>(B, n, VR, P, TR, nb, g, bl, bl + 1, 0, -1, true, 0, bk.start + bk.len);
+      for (int e = threadIdx.x; e < bk.k * bk.len; e += kBT) {
+        const int c = e / bk.len, r = e % bk.len;  // B(i0+r, i0+c) = 0 below the diagonal
+        if (r > c) B[PHT_IDX(bk.start + r, bk.start + c, n)] = 0.0;
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+    // right application to A and Z (ab2rht.f90:371-467)
+    apply_wg<kRightN>(A, n, VR, P, TR, nb, g, 0, nblocks, 0, n, true, 0, n);
+    apply_wg<kRightN>(Z, n, VR, P, TR, nb, g, 0, nblocks, 0, n, true, 0, n);
+  }
+}
+
+// ---- stage 2 -------------------------------------------------------------
+
+// Far updates of one sweep (rht2ht.f90:1019-1325) for compile-time (R, NQ):
+// rows of A, B above each block's window and all rows of Q, Z (right
+// rotations; Q takes the transposed ones), then columns of A, B right of
+// the window (left rotations).  A thread keeps a window of R entries in
+// registers and slides it by one per column ic; the block's rotations are
+// staged in LDS.
+template <int R, int NQ>
+__device__ __noinline__ void far_updates_wg(double* sm, double* A, double* B, double* Q, double* Z, int n,
+                               int s, int nblocks, const double* CU, const double* SU,
+                               const double* CV, const double* SV) {
+  constexpr int LDR = R + NQ;
+  double2* cs = reinterpret_cast<double2*>(sm);  // [NQ][R]
+  const int tid = threadIdx.x;
+  // rows: which 0 A, 1 B (partial), 2 Q, 3 Z; a pass covers kBT rows
+  for (int which = 0; which < 4; ++which) {
+    double* M = which == 0 ? A : which == 1 ? B : which == 2 ? Q : Z;
+    const double* Ct = which == 2 ? CU : CV;
+    const double* St = which == 2 ? SU : SV;
+    const double sgn = which == 2 ? -1.0 : 1.0;
+    for (int r0 = 0; r0 < n; r0 += kBT) {
+      const int i = r0 + tid;
+      const bool valid = i < n;
+      for (int ib = nblocks; ib >= 1; --ib) {
+        const int cw0 = s + (ib - 1) * R;  // 0-based first window column
+        const int W = min(R + NQ - 1, n - cw0);
+        if (W <= 0) continue;
+        if (which < 2 && r0 + 1 >= max(s + 1, s + 1 + (ib - 1) * R)) continue;
+        __syncthreads();  // the previous block's coefficients are consumed
+        for (int e = tid; e < NQ * R; e += kBT) {
+          const int icm = e / R, ig = e % R;
+          const size_t ti = (size_t)(ib - 1) * LDR * NQ + (size_t)icm * LDR + (icm + ig - 1);
+          cs[e] = (ig >= 1) ? make_double2(Ct[ti], sgn * St[ti]) : make_double2(1.0, 0.0);
+        }
+        __syncthreads();
+        double v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = (valid && k < W) ? M[PHT_IDX(i, cw0 + k, n)] : 0.0;
+        for (int icm = 0; icm < NQ; ++icm) {
+          bool act = valid;
+          if (which < 2) act = act && (i + 1 < max(s + 1, s + 1 + (ib - NQ + icm) * R));
+          if (act) {
+#pragma unroll
+            for (int ig = R - 1; ig >= 1; --ig) {
+              const double2 c = cs[icm * R + ig];
+              const double x = v[ig], y = v[ig - 1];
+              v[ig] = c.x * x + c.y * y;
+              v[ig - 1] = c.x * y - c.y * x;
+            }
+          }
+          if (valid && icm < W) M[PHT_IDX(i, cw0 + icm, n)] = v[0];
+#pragma unroll
+          for (int k = 0; k < R - 1; ++k) v[k] = v[k + 1];
+          v[R - 1] = (valid && R + icm < W) ? M[PHT_IDX(i, cw0 + R + icm, n)] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < R - 1; ++k)
+          if (valid && NQ + k < W) M[PHT_IDX(i, cw0 + NQ + k, n)] = v[k];
+      }
+    }
+  }
+  __threadfence_block();
+  // columns: which 0 A, 1 B; a pass covers kBT columns
+  for (int which = 0; which < 2; ++which) {
+    double* M = which == 0 ? A : B;
+    for (int c0 = 0; c0 < n; c0 += kBT) {
+      const int j = c0 + tid;
+      for (int ib = nblocks; ib >= 1; --ib) {
+        const int istop = (which == 0) ? ((ib == 1) ? s + NQ - 1 : s + (ib - 2) * R + NQ)
+                                       : s + (ib - 1) * R + R + NQ - 1;  // 1-based
+        if (c0 + kBT <= istop) continue;
+        const int rw0 = s + (ib - 1) * R;  // 0-based first row
+        const int W = min(R + NQ - 1, n - rw0);
+        if (W <= 0) continue;
+        __syncthreads();
+        for (int e = tid; e < NQ * R; e += kBT) {
+          const int icm = e / R, ig = e % R;
+          const size_t ti = (size_t)(ib - 1) * LDR * NQ + (size_t)icm * LDR + (icm + ig - 1);
+          cs[e] = (ig >= 1) ? make_double2(CU[ti], -SU[ti]) : make_double2(1.0, 0.0);
+        }
+        __syncthreads();
+        const bool act = (j < n) && (j + 1 > istop);
+        if (!act) continue;
+        double v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = (k < W) ? M[PHT_IDX(rw0 + k, j, n)] : 0.0;
+        for (int icm = 0; icm < NQ; ++icm) {
+#pragma unroll
+          for (int ig = R - 1; ig >= 1; --ig) {
+            const double2 c = cs[icm * R + ig];
+            const double x = v[ig], y = v[ig - 1];
+            v[ig] = c.x * x + c.y * y;
+            v[ig - 1] = c.x * y - c.y * x;
+          }
+          if (icm < W) M[PHT_IDX(rw0 + icm, j, n)] = v[0];
+#pragma unroll
+          for (int k = 0; k < R - 1; ++k) v[k] = v[k + 1];
+          v[R - 1] = (R + icm < W) ? M[PHT_IDX(rw0 + R + icm, j, n)] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < R - 1; ++k)
+          if (NQ + k < W) M[PHT_IDX(rw0 + NQ + k, j, n)] = v[k];
+      }
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+}
+
+__device__ __noinline__ void stage2_wg(double* sm, double* A, double* B, double* Q, double* Z, int n, int r,
+                          int nq, double* ws) {
+  using namespace s2;
+  const int ldr = r + nq;
+  const int nbl = (n - 1) / r + ((n - 1) % r != 0);
+  const size_t tsz = (size_t)ldr * nq * nbl;
+  double* CU = ws;
+  double* SU = CU + tsz;
+  double* CV = SU + tsz;
+  double* SV = CV + tsz;
+  double* Mw = SV + tsz;
+  // LDS arena of the chase steps
+  double(*Bb)[kBRmax + 1] = reinterpret_cast<double(*)[kBRmax + 1]>(sm);
+  double* lc = sm + kBRmax * (kBRmax + 1);
+  double* ls = lc + kBRmax;
+  double* rc = ls + kBRmax;
+  double* rs = rc + kBRmax;
+  double* xa = rs + kBRmax;
+  double* xb = xa + kMaxLdr;
+  double* red = xb + kMaxLdr;
+  double* Ml = red + 2 * (kBT / 128) * 128;
+  for (int s = 1; s <= n - 2; s += nq) {
+    const int nblocks = (n - s) / r + ((n - s) % r != 0);
+    const size_t tz = (size_t)ldr * nq * nblocks, msz = (size_t)ldr * ldr;
+    for (size_t e = threadIdx.x; e < tz; e += kBT) {
+      CU[e] = 1.0;
+      CV[e] = 1.0;
+      SU[e] = 0.0;
+      SV[e] = 0.0;
+    }
+    for (size_t e = threadIdx.x; e < msz * nblocks; e += kBT) {
+      const size_t k = e % msz;
+      Mw[e] = ((k % ldr) == (k / ldr)) ? 1.0 : 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    // generation in the reference's order (rht2ht.f90:787-957)
+    for (int ic = 1; ic <= nq; ++ic) {
+      const int jc = s + ic - 1;
+      for (int ib = 1;; ++ib) {
+        int col, row0;
+        if (ib == 1) {
+          if (jc > n) break;
+          col = jc;
+          row0 = jc + 1;
+        } else {
+          col = jc + 1 + (ib - 2) * r;
+          if (col > n) break;
+          row0 = col + r;
+        }
+        chase_step<kBRmax, kBT>(A, n, B, n, n, r, nq, s, ic, ib, col, row0, CU, SU, CV, SV, ldr,
+                                Mw, true, Bb, lc, ls, rc, rs, xa, xb, red, Ml);
+        __threadfence_block();
+        __syncthreads();
+      }
+    }
+    // far updates of the sweep
+    bool done = false;
+#define PHT_FAR(RR, NN)                                                                      \
+    if (!done && r == RR && nq == NN) {                                                      \
+      far_updates_wg<RR, NN>(sm, A, B, Q, Z, n, s, nblocks, CU, SU, CV, SV);                 \
+      done = true;                                                                           \
+    }
+    PHT_BATCH_PAIRS(PHT_FAR)
+#undef PHT_FAR
+  }
+}
+
+__global__ __launch_bounds__(kBT) void k_batched(const BatchPencil* __restrict__ pencils,
+                                                 int count, int* next, double* Ap, double* Bp,
+                                                 double* Qp, double* Zp, double* ws_all,
+                                                 size_t ws_per_wg) {
+  extern __shared__ double sm[];
+  __shared__ int cur;
+  double* ws = ws_all + (size_t)blockIdx.x * ws_per_wg;
+  for (;;) {
+    if (threadIdx.x == 0) cur = atomicAdd(next, 1);
+    __syncthreads();
+    const int idx = cur;
+    __syncthreads();  // every thread has read cur before it is replaced
+    if (idx >= count) break;
+    const BatchPencil pc = pencils[idx];
+    double* A = Ap + pc.offset;
+    double* B = Bp + pc.offset;
+    double* Q = Qp + pc.offset;
+    double* Z = Zp + pc.offset;
+    if (pc.n > pc.r + 1) stage1_wg(sm, A, B, Q, Z, pc.n, pc.r, pc.p, ws);
+    if (pc.n > 2) stage2_wg(sm, A, B, Q, Z, pc.n, pc.r, pc.nq, ws);
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+__global__ void k_batch_reset(int* next) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *next = 0;
+}
+}  // namespace
+
+bool batched_supported(int n, int r, int p, int nq) {
+  if (n < 1 || r < 1 || r > kBRmax || p < 2 || p * r > 128) return false;
+  bool ok = false;
+#define PHT_OK(RR, NN) ok = ok || (r == RR && nq == NN);
+  PHT_BATCH_PAIRS(PHT_OK)
+#undef PHT_OK
+  return ok && s2_sweep_lds(r, nq) != 0;
+}
+
+size_t batched_ws_doubles(int n, int r, int p, int nq) {
+  const size_t P = (size_t)p * r;
+  const size_t nbmax = (n > r) ? (size_t)((n - r + (p - 1) * r - 1) / ((p - 1) * r) + 1) : 1;
+  const size_t s1 = 2 * nbmax * P * r + 2 * nbmax * r * r;
+  const size_t ldr = (size_t)r + nq;
+  const size_t nbl = (size_t)((n - 1) / r + 1);
+  const size_t s2 = 4 * ldr * nq * nbl + ldr * ldr * nbl;
+  return (s1 > s2 ? s1 : s2) + 64;
+}
+
+hipError_t batched_launch(hipStream_t st, const BatchPencil* dpencils, int count, int* dnext,
+                          double* Ap, double* Bp, double* Qp, double* Zp, double* ws,
+                          size_t ws_per_wg, int grid) {
+  static bool attr = false;
+  const size_t lds = s1::kRqLds;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_batched,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, dnext);
+  hipLaunchKernelGGL(k_batched, dim3(grid), dim3(kBT), lds, st, dpencils, count, dnext, Ap, Bp,
+                     Qp, Zp, ws, ws_per_wg);
+  return hipGetLastError();
+}
+
+}  // namespace pht

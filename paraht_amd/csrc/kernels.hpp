@@ -87,4 +87,21 @@ hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
                          int ldb, const S2Params& P, const double* CU,
                          const double* SU);
 
+// ---- batched reduction of many small pencils ----
+// One pencil of a batch: its n x n column-major matrices (ld = n) start at
+// element `offset` of each of the A, B, Q, Z pools.
+struct BatchPencil {
+  long long offset;
+  int n, r, p, nq;
+};
+// (n, r, p, nq) that the batched kernel handles.
+bool batched_supported(int n, int r, int p, int nq);
+// Workspace doubles one workgroup needs for a pencil.
+size_t batched_ws_doubles(int n, int r, int p, int nq);
+// Resets the work queue counter and launches the persistent kernel (grid
+// workgroups, ws_per_wg doubles of workspace each).
+hipError_t batched_launch(hipStream_t st, const BatchPencil* dpencils, int count, int* dnext,
+                          double* Ap, double* Bp, double* Qp, double* Zp, double* ws,
+                          size_t ws_per_wg, int grid);
+
 }  // namespace pht

@@ -45,6 +45,9 @@ struct paraht_ctx {
   double k_ms[PARAHT_NKERNELS] = {0}, k_flops[PARAHT_NKERNELS] = {0},
          k_bytes[PARAHT_NKERNELS] = {0};
   long long k_n[PARAHT_NKERNELS] = {0};
+  // the batched path's captured launch and the arguments it was captured for
+  hipGraphExec_t batch_exec = nullptr;
+  std::vector<long long> batch_key;
 };
 
 namespace {
@@ -575,6 +578,97 @@ int run_host(Which w, int n, double* A, int lda, double* B, int ldb, double* Q,
 }  // namespace
 
 // ------------------------------------------------------------------------
+// Batched path
+// ------------------------------------------------------------------------
+namespace {
+
+// size-bucket table of (r, p, nq) for the batched path, from
+// paraht_amd.batched.autotune on MI355X over 2048 pencils with n uniform in
+// [32, 512] (buckets n <= 96, 192, 320, 512; every bucket chose the widest
+// band, the largest blocks and the most columns per sweep)
+void batch_bucket(int n, int& r, int& p, int& nq) {
+  (void)n;
+  r = 16;
+  p = 8;
+  nq = 16;
+}
+
+int batched(paraht_ctx* c, int count, const paraht_pencil* pencils, double* dA, double* dB,
+            double* dQ, double* dZ, int flags) {
+  std::vector<pht::BatchPencil> q;
+  q.reserve(count);
+  size_t wsmax = 64;
+  for (int i = 0; i < count; ++i) {
+    const paraht_pencil& pc = pencils[i];
+    if (pc.n < 0 || pc.offset < 0 || pc.r < 0 || pc.p < 0 || pc.nq < 0) return -3;
+    if (pc.n == 0) continue;
+    int r = pc.r, p = pc.p, nq = pc.nq;
+    if (r == 0 || p == 0 || nq == 0) {
+      int dr, dp, dn;
+      batch_bucket(pc.n, dr, dp, dn);
+      if (r == 0) r = dr;
+      if (p == 0) p = dp;
+      if (nq == 0) nq = dn;
+    }
+    if (!pht::batched_supported(pc.n, r, p, nq)) {
+      if (verbose())
+        fprintf(stderr, "paraht: batched pencil %d (n=%d r=%d p=%d nq=%d) unsupported\n", i,
+                pc.n, r, p, nq);
+      return PARAHT_ERR_UNSUPPORTED;
+    }
+    q.push_back({pc.offset, pc.n, r, p, nq});
+    wsmax = std::max(wsmax, pht::batched_ws_doubles(pc.n, r, p, nq));
+  }
+  if (q.empty()) return 0;
+  // the work queue: largest first (longest-processing-time order)
+  std::stable_sort(q.begin(), q.end(), [](const pht::BatchPencil& a, const pht::BatchPencil& b) {
+    return a.n > b.n;
+  });
+  const int nq_ = (int)q.size();
+  const int grid = std::min(nq_, c->cu_count);
+  const size_t pbytes = ((nq_ * sizeof(pht::BatchPencil) + 255) / 256) * 256;
+  if (int e = ensure_ws(c, pbytes + 256 + (size_t)grid * wsmax * sizeof(double))) return e;
+  auto* dP = (pht::BatchPencil*)c->ws;
+  int* dnext = (int*)((char*)c->ws + pbytes);
+  double* wsb = (double*)((char*)c->ws + pbytes + 256);
+  hipStream_t st = c->stream;
+  CK(hipMemcpyAsync(dP, q.data(), nq_ * sizeof(pht::BatchPencil), hipMemcpyHostToDevice, st));
+  CK(hipStreamSynchronize(st));  // q is a host temporary
+  if (flags & PARAHT_BATCH_GRAPH) {
+    std::vector<long long> key = {nq_, (long long)(uintptr_t)dA, (long long)(uintptr_t)dB,
+                                  (long long)(uintptr_t)dQ, (long long)(uintptr_t)dZ,
+                                  (long long)(uintptr_t)c->ws, (long long)wsmax, grid};
+    for (const auto& b : q) key.insert(key.end(), {b.offset, b.n, b.r, b.p, b.nq});
+    if (!c->batch_exec || c->batch_key != key) {
+      if (c->batch_exec) {
+        CK(hipGraphExecDestroy(c->batch_exec));
+        c->batch_exec = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      CK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const hipError_t le = pht::batched_launch(st, dP, nq_, dnext, dA, dB, dQ, dZ, wsb, wsmax, grid);
+      const hipError_t ce = hipStreamEndCapture(st, &g);
+      if (le != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(le, "batched capture", __LINE__);
+      }
+      CK(ce);
+      const hipError_t ie = hipGraphInstantiate(&c->batch_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      CK(ie);
+      c->batch_key = key;
+    }
+    CK(hipGraphLaunch(c->batch_exec, st));
+  } else {
+    CK(pht::batched_launch(st, dP, nq_, dnext, dA, dB, dQ, dZ, wsb, wsmax, grid));
+  }
+  CK(hipStreamSynchronize(st));
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
 // C-ABI
 // ------------------------------------------------------------------------
 extern "C" {
@@ -608,6 +702,7 @@ int paraht_ctx_destroy(paraht_ctx* ctx) {
   if (!ctx) return -1;
   (void)hipSetDevice(ctx->device);
   if (ctx->own) (void)hipStreamSynchronize(ctx->own);
+  if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
   if (ctx->ws) (void)hipFree(ctx->ws);
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->own) (void)hipStreamDestroy(ctx->own);
@@ -714,6 +809,29 @@ int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
   if (p < 2) return -12;
   if (nq < 1) return -13;
   return run_dev(ctx, kBoth, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, p, nq, flags);
+}
+
+int paraht_batched_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils, double* dA,
+                       double* dB, double* dQ, double* dZ, int flags) {
+  if (!ctx) return -1;
+  if (count < 0) return -2;
+  if (count == 0) return 0;
+  if (!pencils) return -3;
+  if (!dA) return -4;
+  if (!dB) return -5;
+  if (!dQ) return -6;
+  if (!dZ) return -7;
+  CK(hipSetDevice(ctx->device));
+  return batched(ctx, count, pencils, dA, dB, dQ, dZ, flags);
+}
+
+int paraht_batched_defaults(int n, int* r, int* p, int* nq) {
+  if (n < 0) return -1;
+  if (!r) return -2;
+  if (!p) return -3;
+  if (!nq) return -4;
+  batch_bucket(n, *r, *p, *nq);
+  return 0;
 }
 
 int paraht_version(void) { return 100; }

@@ -1,0 +1,382 @@
+// Device code of the register-resident stage-1 generators for blocks of at most 128 rows and
+// 64 reflectors (P = p*nb <= 128, nb <= 64), the shapes of the benchmark
+// configurations (nb = 64, p = 2; nb = 16, p = 8).  Same mathematics as the
+// generic kernels in stage1.hip (DGEQR2 / DGERQ2 + DORGRQ + DGELQ2 / DLARFT
+// semantics, src/ab2rht.f90:105-116,290-311); the difference is data
+// placement and control flow.
+//  * A workgroup of 1024 threads holds the block in registers with G = 8 or
+//    16 threads per row (column): every thread keeps 16 (8) entries, element
+//    j of a thread in group position q being index G*j + q, so the lanes that
+//    read one LDS vector touch consecutive words.  Building a reflector (the
+//    serial part of every step) costs a thread 8-16 entries, not a row.
+//  * Every reflector is stored zero-padded to 128 entries in its own LDS row,
+//    which is both the broadcast buffer of its update and the explicit V of
+//    the block (one barrier per reflector, no masks in the update loops).
+//  * The work of a reflector is bounded by skipping whole chunks on a
+//    workgroup-uniform test instead of per-element predicates.
+//  * G = V^T V runs on FP64 MFMA (one 16x16 tile per wave); T comes from the
+//    DLARFT forward recurrence T(0:j-1,j) = -tau_j T(0:j-1,0:j-1) G(0:j-1,j),
+//    each lane of one wave owning a row of T (no barriers).
+#pragma once
+#include "device_util.hpp"
+#include "kernels.hpp"
+
+namespace pht {
+namespace s1 {
+constexpr int kT = 1024;
+constexpr int kLd = 129;  // LDS row length of stored reflectors (odd: spreads banks)
+
+// x from another lane of the same DPP row (16 lanes), by DPP control CTRL
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// sum over aligned groups of G = 2, 4, 8 or 16 lanes with DPP swizzles (no
+// LDS round trip): quad_perm xor 1, quad_perm xor 2, then the half-row
+// mirror pairs the two quads of an 8-lane half and the row mirror the two
+// halves of a 16-lane row
+template <int G>
+__device__ __forceinline__ double grp_sum(double v) {
+  static_assert(G == 2 || G == 4 || G == 8 || G == 16, "DPP group");
+  v += dpp_d<0xB1>(v);                // quad_perm(1,0,3,2)
+  if (G > 2) v += dpp_d<0x4E>(v);     // quad_perm(2,3,0,1)
+  if (G > 4) v += dpp_d<0x141>(v);    // row_half_mirror
+  if (G > 8) v += dpp_d<0x140>(v);    // row_mirror
+  return v;
+}
+template <int G>
+__device__ __forceinline__ double grp_max(double v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// ||x|| over a group of G lanes holding NJ values each, entries selected by
+// `sel`; the scaled evaluation runs only when the plain sum of squares
+// over/underflowed.
+template <int G, int NJ, typename Sel>
+__device__ __forceinline__ double grp_norm(const double* X, Sel sel) {
+  double ss = 0.0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) ss += sel(j) ? X[j] * X[j] : 0.0;
+  ss = grp_sum<G>(ss);
+  if (ss == 0.0 || (ss > 1e-290 && ss < 1e290)) return sqrt(ss);
+  double mx = 0.0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) mx = fmax(mx, sel(j) ? fabs(X[j]) : 0.0);
+  mx = grp_max<G>(mx);
+  if (mx == 0.0) return 0.0;
+  double s2 = 0.0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const double t = sel(j) ? X[j] / mx : 0.0;
+    s2 += t * t;
+  }
+  return mx * sqrt(grp_sum<G>(s2));
+}
+
+// G(x,y) = v_x . v_y (x < y < k) for reflectors stored as zero-padded rows of
+// Vs (kLd apart, 128 entries), then T (kfull x kfull, ld ldt, zeros below the
+// diagonal and beyond k) of H = H(0) ... H(k-1) = I - V T V^T.  Called by all
+// kT threads after a barrier that completed Vs and tau.
+__device__ inline void gram_and_t(const double* Vs, int k, const double* tau, double* G,
+                           double* Tl, double* Tout, int ldt, int kfull) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int l16 = lane & 15, q = lane >> 4;
+  {
+    // one 16x16 tile of G per wave: rows 16*I.., columns 16*J.. (I <= J)
+    const int I = wave >> 2, J = wave & 3;
+    if (I <= J && 16 * I < k) {
+      const double* va = Vs + (16 * I + l16) * kLd;
+      const double* vb = Vs + (16 * J + l16) * kLd;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+      for (int k0 = 0; k0 < 128; k0 += 4) acc = mfma16(va[k0 + q], vb[k0 + q], acc);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) G[(16 * I + 4 * i + q) * 65 + 16 * J + l16] = acc[i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    // lane a computes row a of T; it reads only its own row and G
+    const int a = threadIdx.x;
+    if (a < k) Tl[a * 65 + a] = tau[a];
+    for (int j = a + 1; j < k; ++j) {
+      double s0 = 0.0, s1 = 0.0;
+      int l = a;
+      for (; l + 1 < j; l += 2) {
+        s0 += Tl[a * 65 + l] * G[l * 65 + j];
+        s1 += Tl[a * 65 + l + 1] * G[(l + 1) * 65 + j];
+      }
+      if (l < j) s0 += Tl[a * 65 + l] * G[l * 65 + j];
+      Tl[a * 65 + j] = -tau[j] * (s0 + s1);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kfull * kfull; e += kT) {
+    const int r = e % kfull, c = e / kfull;
+    Tout[PHT_IDX(r, c, ldt)] = (r <= c && c < k) ? Tl[r * 65 + c] : 0.0;
+  }
+}
+// ---------------------------------------------------------------------------
+// Left generation of one panel (ab2rht.f90:93-118), blocks bottom-up.
+// Thread t owns column t/16 of the block, rows 16*j + t%16 (j < 8).
+__device__ inline void panel_qr_body(
+    double* sm, double* __restrict__ A, int lda, int n, int jb, int nb, int p, int nblocks,
+    double* __restrict__ Vt, int ldv, long long vstride, double* __restrict__ Tt,
+    int ldt, long long tstride) {
+  constexpr int NJ = 8;
+  double* Vc = sm;               // [64][kLd]: reflector i in row i
+  double* G = Vc + 64 * kLd;     // [64][65]
+  double* Tl = G + 64 * 65;      // [64][65]
+  double* tau = Tl + 64 * 65;    // [64]
+  const int tid = threadIdx.x;
+  const int col = tid >> 4, q = tid & 15;
+  const int P = p * nb;
+
+  for (int bl = 0; bl < nblocks; ++bl) {
+    const int i0 = jb + nb + (nblocks - 1 - bl) * (p - 1) * nb;
+    const int len = min(n - i0, P);
+    const int k = min(nb, len);
+    double X[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int r = 16 * j + q;
+      X[j] = (col < nb && r < len) ? A[PHT_IDX(i0 + r, jb + col, lda)] : 0.0;
+    }
+    // DGEQR2: reflector i from column i (16 lanes), applied to the columns
+    // right of it; it is zero above row i.
+    for (int i = 0; i < k; ++i) {
+      double* vi = Vc + i * kLd;
+      if (col == i) {
+        double al = 0.0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) al += (16 * j + q == i) ? X[j] : 0.0;
+        al = grp_sum<16>(al);
+        const double xnorm = grp_norm<16, NJ>(X, [&](int j) { return 16 * j + q > i; });
+        const Refl R = make_refl(al, xnorm);
+        const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int r = 16 * j + q;
+          const double v = (r > i) ? X[j] * sc : (r == i ? 1.0 : 0.0);
+          vi[r] = v;
+          X[j] = (r > i) ? v : (r == i ? R.beta : X[j]);
+        }
+        if (q == 0) tau[i] = R.tau;
+      }
+      __syncthreads();
+      const double t = tau[i];
+      if (col > i && col < nb && t != 0.0) {
+        double w = 0.0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          if (16 * j + 15 < i) continue;  // entries above the reflector
+          w += X[j] * vi[16 * j + q];
+        }
+        w = grp_sum<16>(w) * t;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          if (16 * j + 15 < i) continue;
+          X[j] -= w * vi[16 * j + q];
+        }
+      }
+    }
+    // R (upper trapezoid) and zeros back into A
+    if (col < nb) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int r = 16 * j + q;
+        if (r < len) A[PHT_IDX(i0 + r, jb + col, lda)] = (r <= col) ? X[j] : 0.0;
+      }
+    }
+    // explicit V to the tape (coalesced along the rows)
+    double* Vb = Vt + (long long)bl * vstride;
+    for (int e = tid; e < len * nb; e += kT) {
+      const int r = e % len, c = e / len;
+      Vb[PHT_IDX(r, c, ldv)] = (c < k) ? Vc[c * kLd + r] : 0.0;
+    }
+    gram_and_t(Vc, k, tau, G, Tl, Tt + (long long)bl * tstride, ldt, nb);
+    // the next block (above) re-reads the R rows just written to A, and
+    // overwrites Vc / tau
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Right generator for one B block of len <= 128 rows (ab2rht.f90:290-311).
+// len > nb: RQ, the first nb rows of its Q, LQ of those (nb reflectors).
+// len <= nb: the len RQ reflectors themselves, applied as H(len-1)...H(0)
+// (the 'B','R' branch, ab2rht.f90:305-309).
+// RQ phase: thread t owns row t/8 of the block, columns 8*j + t%8 (j < 16).
+// Y / LQ phase: thread t owns row t/16 of Y = Q(0:nb-1,:), columns 16*j + t%16.
+__device__ inline void rq_gen_body(
+    double* sm, const double* __restrict__ B, int ldb, int i0, int len, int nb,
+    double* __restrict__ Vout, int ldv, double* __restrict__ Tout, int ldt) {
+  double* Vr = sm;                  // [128][kLd]: RQ reflector i in row i
+  double* taus = Vr + 128 * kLd;    // [128]
+  double* tau2 = taus + 128;        // [64]
+  double* Ys = Vr;                  // [64][kLd]: LQ reflectors, after the Y phase
+  double* G = Vr + 64 * kLd;        // [64][65]
+  double* Tl = G + 64 * 65;         // [64][65] (ends inside taus, dead by then)
+  const int tid = threadIdx.x;
+
+  // ---- DGERQ2 on the len x len block: reflector i from row i, zero right of
+  // column i, applied to the rows above it.
+  {
+    constexpr int NJ = 16;
+    const int row = tid >> 3, h = tid & 7;
+    double X[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int c = 8 * j + h;
+      X[j] = (row < len && c < len) ? B[PHT_IDX(i0 + row, i0 + c, ldb)] : 0.0;
+    }
+    for (int i = len - 1; i >= 0; --i) {
+      double* vi = Vr + i * kLd;
+      if (row == i) {
+        double al = 0.0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) al += (8 * j + h == i) ? X[j] : 0.0;
+        al = grp_sum<8>(al);
+        const double xnorm = grp_norm<8, NJ>(X, [&](int j) { return 8 * j + h < i; });
+        const Refl R = make_refl(al, xnorm);
+        const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int c = 8 * j + h;
+          vi[c] = (c < i) ? X[j] * sc : (c == i ? 1.0 : 0.0);
+        }
+        if (h == 0) taus[i] = R.tau;
+      }
+      __syncthreads();
+      const double t = taus[i];
+      if (row < i && t != 0.0) {
+        double w0 = 0.0, w1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < NJ; j += 2) {
+          if (8 * j > i) continue;  // entries right of the reflector
+          w0 += X[j] * vi[8 * j + h];
+          w1 += X[j + 1] * vi[8 * j + 8 + h];
+        }
+        const double w = grp_sum<8>(w0 + w1) * t;
+#pragma unroll
+        for (int j = 0; j < NJ; j += 2) {
+          if (8 * j > i) continue;
+          X[j] -= w * vi[8 * j + h];
+          X[j + 1] -= w * vi[8 * j + 8 + h];
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  if (len <= nb) {
+    // B Q^T = B H(len-1) ... H(0): the reflectors in reverse order form a
+    // forward product; they go to rows 64.. of the buffer (free here)
+    double* Vb = Vr + 64 * kLd;
+    for (int e = tid; e < len * 128; e += kT) {
+      const int j = e / 128, c = e % 128;
+      Vb[j * kLd + c] = Vr[(len - 1 - j) * kLd + c];
+      if (c == 0) tau2[j] = taus[len - 1 - j];
+    }
+    __syncthreads();
+    for (int e = tid; e < len * nb; e += kT) {
+      const int c = e % len, j = e / len;
+      Vout[PHT_IDX(c, j, ldv)] = (j < len) ? Vb[j * kLd + c] : 0.0;
+    }
+    // G and T go to rows 0..64 of the buffer (dead once copied); T's last
+    // row reaches the first reflector row only after the Gram phase read it
+    double* G2 = Vr;
+    gram_and_t(Vb, len, tau2, G2, G2 + 64 * 65, Tout, ldt, nb);
+    return;
+  }
+
+  // ---- Y = E^T H(0) H(1) ... H(len-1): the first nb rows of the RQ's Q
+  constexpr int NJ = 8;
+  const int a = tid >> 4, q = tid & 15;
+  double Y[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) Y[j] = (a < nb && 16 * j + q == a) ? 1.0 : 0.0;
+  if (a < nb) {
+    for (int i = 1; i < len; ++i) {  // H(0) is the identity (length-1 reflector)
+      const double t = taus[i];
+      if (t == 0.0) continue;
+      const double* vi = Vr + i * kLd;
+      double w = 0.0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        if (16 * j > i) continue;
+        w += Y[j] * vi[16 * j + q];
+      }
+      w = grp_sum<16>(w) * t;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        if (16 * j > i) continue;
+        Y[j] -= w * vi[16 * j + q];
+      }
+    }
+  }
+  __syncthreads();  // Ys overwrites Vr from here on
+
+  // ---- DGELQ2 on Y (nb x len): reflector i from row i, zero left of column
+  // i, applied to the rows below it.  Its LDS row is the explicit V row.
+  for (int i = 0; i < nb; ++i) {
+    double* vi = Ys + i * kLd;
+    if (a == i) {
+      double al = 0.0, ss = 0.0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int c = 16 * j + q;
+        al += (c == i) ? Y[j] : 0.0;
+        ss += (c > i) ? Y[j] * Y[j] : 0.0;  // rows of an orthogonal factor: |entries| <= 1
+      }
+      al = grp_sum<16>(al);
+      const double xnorm = sqrt(grp_sum<16>(ss));
+      const Refl R = make_refl(al, xnorm);
+      const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int c = 16 * j + q;
+        vi[c] = (c > i) ? Y[j] * sc : (c == i ? 1.0 : 0.0);
+      }
+      if (q == 0) tau2[i] = R.tau;
+    }
+    __syncthreads();
+    const double t = tau2[i];
+    if (a > i && a < nb && t != 0.0) {
+      double w = 0.0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        if (16 * j + 15 < i) continue;  // entries left of the reflector
+        w += Y[j] * vi[16 * j + q];
+      }
+      w = grp_sum<16>(w) * t;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        if (16 * j + 15 < i) continue;
+        Y[j] -= w * vi[16 * j + q];
+      }
+    }
+  }
+  // explicit V (len x nb), coalesced along the rows
+  for (int e = tid; e < len * nb; e += kT) {
+    const int c = e % len, r = e / len;
+    Vout[PHT_IDX(c, r, ldv)] = Ys[r * kLd + c];
+  }
+  gram_and_t(Ys, nb, tau2, G, Tl, Tout, ldt, nb);
+}
+
+
+constexpr size_t kPanelLds = (size_t)(64 * kLd + 2 * 64 * 65 + 64) * sizeof(double);
+constexpr size_t kRqLds = (size_t)(128 * kLd + 128 + 64) * sizeof(double);
+static_assert(kPanelLds <= 160 * 1024 && kRqLds <= 160 * 1024, "LDS budget");
+static_assert(64 * kLd + 2 * 64 * 65 <= 128 * kLd + 128, "Gram/T alias in the RQ kernel");
+
+}  // namespace s1
+}  // namespace pht

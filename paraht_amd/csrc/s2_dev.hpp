@@ -1,0 +1,309 @@
+// Device code of the stage-2 Givens chase (rht2ht_givens semantics,
+// src/rht2ht.f90:696-1344), shared by the pipelined sweep kernel
+// (stage2.hip) and the batched persistent kernel.
+#pragma once
+#include "device_util.hpp"
+#include "kernels.hpp"
+
+namespace pht {
+namespace s2 {
+
+__device__ __forceinline__ size_t tape_idx(int loc1, int ic1, int ib1, int ldr, int nq) {
+  // 1-based (local row, ic, ib) -> offset in CU, SU, CV, SV(ldr, nq, nblocks)
+  return (size_t)(loc1 - 1) + (size_t)(ic1 - 1) * ldr + (size_t)(ib1 - 1) * ldr * nq;
+}
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+
+constexpr int kMaxLdr = 160;  // r + nq (host checks)
+
+// Givens rotation from (f, g) when |r| = d is already known (d > 0):
+// [c s; -s c][f; g] = [r; 0], c >= 0, r with the sign of f (f == 0: c = 0,
+// s = sign g) -- the conventions of dlartg_d.
+__device__ __forceinline__ void rot_from_norm(double f, double g, double d,
+                                              double& c, double& sn) {
+  if (g == 0.0) {
+    c = 1.0;
+    sn = 0.0;
+  } else if (f == 0.0) {
+    c = 0.0;
+    sn = copysign(1.0, g);
+  } else {
+    const double rinv = 1.0 / d;
+    c = fabs(f) * rinv;
+    sn = ((f < 0.0) ? -g : g) * rinv;  // s = g / r with r = sign(f) d
+  }
+}
+
+// Right rotations (rc, rs)[ig], ig = nb-1..1, of one step applied to row rr0
+// (0-based) of Mx on the step's columns R0..R0+nb-1.
+template <int RMAX>
+__device__ __forceinline__ void rotate_row(double* __restrict__ Mx, int ld, int rr0, int R0,
+                                           int nb, const double* rc, const double* rs) {
+  double seg[RMAX];
+#pragma unroll
+  for (int j = 0; j < RMAX; ++j) seg[j] = (j < nb) ? Mx[PHT_IDX(rr0, R0 + j, ld)] : 0.0;
+  double carry = 0.0;
+#pragma unroll
+  for (int ig = RMAX - 1; ig >= 1; --ig) {
+    if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
+    if (ig < nb) {
+      if (ig == nb - 1) carry = seg[ig];
+      const double c = rc[ig], sn = rs[ig];
+      const double y = seg[ig - 1];
+      seg[ig] = c * carry + sn * y;
+      carry = c * y - sn * carry;
+    }
+  }
+  seg[0] = carry;
+#pragma unroll
+  for (int j = 0; j < RMAX; ++j)
+    if (j < nb) Mx[PHT_IDX(rr0, R0 + j, ld)] = seg[j];
+}
+
+// Right chain on the B block Bb (one wave, lane = block row): rotations
+// ig = nb-1..1 on columns (ig-1, ig) zero B(ig, ig-1), the pivot values being
+// read from lane ig.  Rotation ig only mixes columns ig-1 and ig, and column
+// ig-1 is untouched before it, so a lane carries one running value (`cur`,
+// its entry in column ig) and streams the original column ig-1 from LDS;
+// column ig is final after rotation ig.  (rc, rs)[ig] receive the rotations.
+template <int RMAX>
+__device__ __forceinline__ void right_chain(double (*Bb)[RMAX + 1], int nb, int lane,
+                                            double* rc, double* rs) {
+  const bool row = lane < nb;
+  double cur = row ? Bb[lane][nb - 1] : 0.0;
+  double myc = 1.0, mys = 0.0;  // the rotation pivoted on this lane
+#pragma unroll 4
+  for (int ig = nb - 1; ig >= 1; --ig) {
+    const double y = row ? Bb[lane][ig - 1] : 0.0;
+    const double f = readlane_d(cur, ig);
+    const double g = readlane_d(y, ig);
+    double c, sn, rr;
+    rot_rsq(f, g, c, sn, rr);
+    const double nx = c * cur + sn * y, ny = c * y - sn * cur;
+    const bool piv = lane == ig, above = lane < ig;
+    if (row) Bb[lane][ig] = piv ? rr : (above ? nx : cur);
+    cur = piv ? 0.0 : (above ? ny : y);
+    myc = piv ? c : myc;
+    mys = piv ? sn : mys;
+  }
+  if (row) Bb[lane][0] = cur;
+  if (lane >= 1 && lane < nb) {
+    rc[lane] = myc;
+    rs[lane] = mys;
+  }
+}
+
+// Left rotations (lc, ls)[ig], ig = nb-1..1, on the entries p[k*st], k < nb,
+// of one LDS column: (a, b) = (p[ig-1], p[ig]) <- (c a + s b, c b - s a).
+// The reads do not depend on the carried value and run ahead of it.
+__device__ __forceinline__ void left_chain_lds(double* p, int st, int nb, const double* lc,
+                                               const double* ls) {
+  double carry = p[(nb - 1) * st];
+#pragma unroll 8
+  for (int ig = nb - 1; ig >= 1; --ig) {
+    const double c = lc[ig], sn = ls[ig];
+    const double a = p[(ig - 1) * st];
+    p[ig * st] = c * carry - sn * a;
+    carry = c * a + sn * carry;
+  }
+  p[0] = carry;
+}
+
+// One step (ic, ib) of a sweep starting at column s (rht2ht.f90:787-957), by
+// all NT threads of a workgroup:
+//   * M of block ib (the product of the deferred left rotations of columns
+//     1..ic-1, identity for ic = 1) is staged in LDS (Ml, ld ldr+1) and
+//     applied to A(:, col) and B(:, row0+r-1) as one mat-vec;
+//   * the left chain zeroes A(row0+1:row0+nb-1, col) (tape CU, SU), its
+//     rotations are applied to the B block and folded into M (written back);
+//   * the right chain restores the B block's triangularity (tape CV, SV) and
+//     its rotations are applied to the window rows: A rows row0..row0+r+nb-1
+//     and, with `above`, A and B rows istart..row0-1.
+// col, row0 are 1-based as in the reference.  LDS: Bb[RMAX][RMAX+1], lc, ls,
+// rc, rs [RMAX], xa, xb [kMaxLdr], red [2][NT/128][128], Ml [ldr][ldr+1].
+template <int RMAX, int NT>
+__device__ inline void chase_step(double* __restrict__ A, int lda, double* __restrict__ B,
+                                  int ldb, int n, int r, int nq, int s, int ic, int ib, int col,
+                                  int row0, double* __restrict__ CU, double* __restrict__ SU,
+                                  double* __restrict__ CV, double* __restrict__ SV, int ldr,
+                                  double* __restrict__ Mw, bool above, double (*Bb)[RMAX + 1],
+                                  double* lc, double* ls, double* rc, double* rs, double* xa,
+                                  double* xb, double* red, double* Ml) {
+  constexpr int kParts = NT / 128;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int ldl = ldr + 1;
+  const size_t msz = (size_t)ldr * ldr;
+  const int nb = imin(r, n - row0 + 1);
+  const int base = s + (ib - 1) * r;  // local row l of block ib = row base+l
+  const int L = imin(ldr - 1, n - base);
+  double* M = Mw + (size_t)(ib - 1) * msz;
+
+  // ---- M of this block into LDS (identity for the first column)
+  for (int e0 = 0; e0 < ldr * ldr; e0 += 16 * NT) {
+    double tmp[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = e0 + tid + q * NT;
+      tmp[q] = (ic > 1 && e < ldr * ldr) ? M[e] : (e % ldr == e / ldr ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = e0 + tid + q * NT;
+      if (e < ldr * ldr) Ml[(e / ldr) * ldl + e % ldr] = tmp[q];
+    }
+  }
+  __syncthreads();
+
+  // ---- deferred left rotations of the sweep's earlier columns:
+  //      A(:, col) and B(:, row0+r-1), local rows 1..L of block ib.
+  //      Thread t: output rows i = t%128 (+128), half h = t/128 of the
+  //      reduction; its loads of M are all in flight together.
+  if (ic > 1 && L > 0) {
+    const bool doB = row0 + r - 1 <= n;
+    for (int t = tid; t < L; t += NT) {
+      xa[t] = A[PHT_IDX(base + t, col - 1, lda)];
+      xb[t] = doB ? B[PHT_IDX(base + t, row0 + r - 2, ldb)] : 0.0;
+    }
+    __syncthreads();
+    const int h = tid >> 7, il = tid & 127;
+    const int jp = (L + kParts - 1) / kParts, j0 = h * jp, j1 = imin(L, j0 + jp);
+    for (int i0 = 0; i0 < L; i0 += 128) {
+      const int i = i0 + il;
+      double ya = 0.0, yb = 0.0;
+      if (i < L) {
+        for (int j = j0; j < j1; ++j) {
+          const double m = Ml[j * ldl + i];
+          ya += m * xa[j];
+          yb += m * xb[j];
+        }
+      }
+      red[(0 * kParts + h) * 128 + il] = ya;
+      red[(1 * kParts + h) * 128 + il] = yb;
+      __syncthreads();
+      if (h == 0 && i < L) {
+        double sa = 0.0, sb = 0.0;
+#pragma unroll
+        for (int q = 0; q < kParts; ++q) {
+          sa += red[(0 * kParts + q) * 128 + il];
+          sb += red[(1 * kParts + q) * 128 + il];
+        }
+        A[PHT_IDX(base + i, col - 1, lda)] = sa;
+        if (doB) B[PHT_IDX(base + i, row0 + r - 2, ldb)] = sb;
+      }
+      __syncthreads();
+    }
+  }
+
+  // the reference leaves the column here when nb < 2 (rht2ht.f90:827,911);
+  // later blocks still owe this column their deferred rotations
+  if (nb >= 2) {
+    const int istart = imax(s + 1, s + 1 + (ib - 1 - nq + ic) * r);
+    const int R0 = row0 - 1, C0 = col - 1;  // 0-based anchors
+
+    // ---- left chain: zero A(row0+1:row0+nb-1, col) (rht2ht.f90:830-835,
+    //      914-919).  The chain's partial results are signed suffix norms of
+    //      the column, so every rotation follows from a suffix sum of
+    //      squares (wave 0, lane = row).
+    if (wave == 0) {
+      const double x = (lane < nb) ? A[PHT_IDX(R0 + lane, C0, lda)] : 0.0;
+      const double mx = wave_max(fabs(x));
+      const double xs = (mx > 0.0) ? x / mx : 0.0;
+      double S = xs * xs;  // inclusive suffix sum over lanes >= lane
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const double t = __shfl_down(S, off, 64);
+        if (lane + off < 64) S += t;
+      }
+      const double Snext = __shfl_down(S, 1, 64);  // suffix from lane+1
+      // g: the value the chain leaves at row row0+lane
+      double g;
+      if (lane >= nb - 1 || Snext == 0.0)
+        g = x;
+      else
+        g = (x < 0.0 ? -1.0 : 1.0) * (sqrt(S) * mx);
+      const double f_above = __shfl_up(x, 1, 64);        // x_{lane-1}
+      const double d_above = __shfl_up(fabs(g), 1, 64);  // |g_{lane-1}|
+      if (lane >= 1 && lane < nb) {
+        double c, sn;
+        rot_from_norm(f_above, g, d_above, c, sn);
+        lc[lane] = c;
+        ls[lane] = sn;
+        const size_t ti = tape_idx(ic + lane - 1, ic, ib, ldr, nq);
+        CU[ti] = c;
+        SU[ti] = sn;
+      }
+      if (lane < nb) A[PHT_IDX(R0 + lane, C0, lda)] = (lane == 0) ? g : 0.0;
+    }
+    // B block into LDS (coalesced by rows; all loads in flight together)
+    {
+      constexpr int PER = (RMAX * RMAX + NT - 1) / NT;
+      double tmp[PER];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * NT;
+        const int k = e % RMAX, j = e / RMAX;
+        tmp[q] = (k < nb && j < nb) ? B[PHT_IDX(R0 + k, R0 + j, ldb)] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * NT;
+        if (e < RMAX * RMAX) Bb[e % RMAX][e / RMAX] = tmp[q];
+      }
+    }
+    __syncthreads();
+
+    // ---- left rotations on the B block (rht2ht.f90:837-840,921-924):
+    //      threads 0..nb-1, one block column each; meanwhile threads
+    //      64.. fold the same rotations into M (local rows ic-1..ic+nb-2)
+    if (tid < nb)
+      left_chain_lds(&Bb[0][tid], RMAX + 1, nb, lc, ls);
+    else if (tid >= 64 && tid < 64 + ldr)
+      left_chain_lds(&Ml[(tid - 64) * ldl + (ic - 1)], 1, nb, lc, ls);
+    __syncthreads();
+    // the updated M rows go back while wave 0 runs the right chain
+    if (wave > 0) {
+      constexpr int PM = (RMAX * kMaxLdr + NT - 64 - 1) / (NT - 64);
+#pragma unroll
+      for (int q = 0; q < PM; ++q) {
+        const int e = (tid - 64) + q * (NT - 64);
+        const int k = e % RMAX, cm = e / RMAX;
+        if (k < nb && cm < ldr) M[(ic - 1) + k + (size_t)cm * ldr] = Ml[cm * ldl + (ic - 1) + k];
+      }
+    }
+
+    // ---- right chain on the B block (rht2ht.f90:843-851,927-935):
+    //      wave 0, lane = block row; the pivot values are read from lane ig
+    if (wave == 0) right_chain<RMAX>(Bb, nb, lane, rc, rs);
+    __syncthreads();
+    for (int e = tid; e < nb * nb; e += NT) {
+      const int k = e % nb, j = e / nb;
+      B[PHT_IDX(R0 + k, R0 + j, ldb)] = Bb[k][j];
+    }
+    if (tid >= 1 && tid < nb) {
+      const size_t ti = tape_idx(ic + tid - 1, ic, ib, ldr, nq);
+      CV[ti] = rc[tid];
+      SV[ti] = rs[tid];
+    }
+
+    // ---- right rotations on the window rows (rht2ht.f90:853-867,937-951):
+    //      A rows istart..row0+r+nb-1 and B rows istart..row0-1; without
+    //      `above` only A rows row0..row0+r+nb-1 (helpers take the rest)
+    {
+      const int aend = imin(n, row0 + r + nb - 1);  // 1-based inclusive
+      const int first = above ? istart : row0;
+      const int nAr = aend - first + 1;
+      const int nBr = above ? row0 - istart : 0;
+      for (int t = tid; t < nAr + nBr; t += NT) {
+        const bool inA = t < nAr;
+        rotate_row<RMAX>(inA ? A : B, inA ? lda : ldb, first - 1 + (inA ? t : t - nAr), R0,
+                         nb, rc, rs);
+      }
+    }
+  }
+}
+
+}  // namespace s2
+}  // namespace pht

@@ -24,26 +24,15 @@
 // rows above the reference's window and columns right of it, plus Q and Z --
 // is done by k_s2_rows / k_s2_cols after the sweep, in the reference's
 // bottom-up block order (rht2ht.f90:1019-1325).
-#include "device_util.hpp"
-#include "kernels.hpp"
+#include "s2_dev.hpp"
 
 namespace pht {
 
 namespace {
 constexpr int kGenThreads = 512;
 constexpr int kTile = 128;
-
-__device__ __forceinline__ size_t tape_idx(int loc1, int ic1, int ib1, int ldr,
-                                           int nq) {
-  // 1-based (local row, ic, ib) -> offset
-  return (size_t)(loc1 - 1) + (size_t)(ic1 - 1) * ldr +
-         (size_t)(ib1 - 1) * ldr * nq;
-}
-
-__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
-__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
-
 }  // namespace
+using namespace s2;
 
 // ---------------------------------------------------------------------------
 // Generation of one sweep (rht2ht.f90:787-957).
@@ -63,9 +52,7 @@ __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 constexpr int kSpinLimit = 1 << 26;
 constexpr int kMaxHelpers = kS2MaxHelpers;
 constexpr int kDone = 1 << 29;
-constexpr int kMaxLdr = 160;                 // r + nq (host checks)
 constexpr int kParts = kGenThreads / 128;    // reduction parts of the mat-vec
-constexpr int kMaxPart = (kMaxLdr + kParts - 1) / kParts;
 
 __global__ void k_s2_prep(double* CU, double* SU, double* CV, double* SV,
                           size_t tsz, double* Mw, int ldr, int nblocks,
@@ -85,99 +72,6 @@ __global__ void k_s2_prep(double* CU, double* SU, double* CV, double* SV,
   }
   for (size_t e = gid; e < (size_t)nq * (1 + kMaxHelpers); e += gsz) progress[e] = 0;
   if (gid == 0) *errflag = 0;
-}
-
-// Givens rotation from (f, g) when |r| = d is already known (d > 0):
-// [c s; -s c][f; g] = [r; 0], c >= 0, r with the sign of f (f == 0: c = 0,
-// s = sign g) -- the conventions of dlartg_d.
-__device__ __forceinline__ void rot_from_norm(double f, double g, double d,
-                                              double& c, double& sn) {
-  if (g == 0.0) {
-    c = 1.0;
-    sn = 0.0;
-  } else if (f == 0.0) {
-    c = 0.0;
-    sn = copysign(1.0, g);
-  } else {
-    const double rinv = 1.0 / d;
-    c = fabs(f) * rinv;
-    sn = ((f < 0.0) ? -g : g) * rinv;  // s = g / r with r = sign(f) d
-  }
-}
-
-// Right rotations (rc, rs)[ig], ig = nb-1..1, of one step applied to row rr0
-// (0-based) of Mx on the step's columns R0..R0+nb-1.
-template <int RMAX>
-__device__ __forceinline__ void rotate_row(double* __restrict__ Mx, int ld, int rr0, int R0,
-                                           int nb, const double* rc, const double* rs) {
-  double seg[RMAX];
-#pragma unroll
-  for (int j = 0; j < RMAX; ++j) seg[j] = (j < nb) ? Mx[PHT_IDX(rr0, R0 + j, ld)] : 0.0;
-  double carry = 0.0;
-#pragma unroll
-  for (int ig = RMAX - 1; ig >= 1; --ig) {
-    if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
-    if (ig < nb) {
-      if (ig == nb - 1) carry = seg[ig];
-      const double c = rc[ig], sn = rs[ig];
-      const double y = seg[ig - 1];
-      seg[ig] = c * carry + sn * y;
-      carry = c * y - sn * carry;
-    }
-  }
-  seg[0] = carry;
-#pragma unroll
-  for (int j = 0; j < RMAX; ++j)
-    if (j < nb) Mx[PHT_IDX(rr0, R0 + j, ld)] = seg[j];
-}
-
-// Right chain on the B block Bb (one wave, lane = block row): rotations
-// ig = nb-1..1 on columns (ig-1, ig) zero B(ig, ig-1), the pivot values being
-// read from lane ig.  Rotation ig only mixes columns ig-1 and ig, and column
-// ig-1 is untouched before it, so a lane carries one running value (`cur`,
-// its entry in column ig) and streams the original column ig-1 from LDS;
-// column ig is final after rotation ig.  (rc, rs)[ig] receive the rotations.
-template <int RMAX>
-__device__ __forceinline__ void right_chain(double (*Bb)[RMAX + 1], int nb, int lane,
-                                            double* rc, double* rs) {
-  const bool row = lane < nb;
-  double cur = row ? Bb[lane][nb - 1] : 0.0;
-  double myc = 1.0, mys = 0.0;  // the rotation pivoted on this lane
-#pragma unroll 4
-  for (int ig = nb - 1; ig >= 1; --ig) {
-    const double y = row ? Bb[lane][ig - 1] : 0.0;
-    const double f = readlane_d(cur, ig);
-    const double g = readlane_d(y, ig);
-    double c, sn, rr;
-    rot_rsq(f, g, c, sn, rr);
-    const double nx = c * cur + sn * y, ny = c * y - sn * cur;
-    const bool piv = lane == ig, above = lane < ig;
-    if (row) Bb[lane][ig] = piv ? rr : (above ? nx : cur);
-    cur = piv ? 0.0 : (above ? ny : y);
-    myc = piv ? c : myc;
-    mys = piv ? sn : mys;
-  }
-  if (row) Bb[lane][0] = cur;
-  if (lane >= 1 && lane < nb) {
-    rc[lane] = myc;
-    rs[lane] = mys;
-  }
-}
-
-// Left rotations (lc, ls)[ig], ig = nb-1..1, on the entries p[k*st], k < nb,
-// of one LDS column: (a, b) = (p[ig-1], p[ig]) <- (c a + s b, c b - s a).
-// The reads do not depend on the carried value and run ahead of it.
-__device__ __forceinline__ void left_chain_lds(double* p, int st, int nb, const double* lc,
-                                               const double* ls) {
-  double carry = p[(nb - 1) * st];
-#pragma unroll 8
-  for (int ig = nb - 1; ig >= 1; --ig) {
-    const double c = lc[ig], sn = ls[ig];
-    const double a = p[(ig - 1) * st];
-    p[ig * st] = c * carry - sn * a;
-    carry = c * a + sn * carry;
-  }
-  p[0] = carry;
 }
 
 // Thread 0 spins (bounded) until *p >= target; the workgroup then acquires.
@@ -256,10 +150,7 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
   __shared__ double xa[kMaxLdr], xb[kMaxLdr];
   __shared__ double red[2][kParts][128];
   extern __shared__ double Ml[];  // M of the current block: M(i, j) at Ml[j*(ldr+1) + i]
-  const int ldl = ldr + 1;
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
   if ((int)blockIdx.x >= nq) {  // helper workgroups follow the nq leaders
     const int hidx = blockIdx.x - nq;
     sweep_helper<RMAX>(A, lda, B, ldb, n, r, nq, s, CV, SV, ldr, progress, hprog, errflag,
@@ -268,7 +159,6 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
   }
   const int ic = blockIdx.x + 1;
   const int jc = s + ic - 1;
-  const size_t msz = (size_t)ldr * ldr;
 
   for (int ib = 1;; ++ib) {
     int col, row0;
@@ -287,173 +177,9 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
       __syncthreads();
       __threadfence();
     }
-    const int nb = imin(r, n - row0 + 1);
-    const int base = s + (ib - 1) * r;  // local row l of block ib = row base+l
-    const int L = imin(ldr - 1, n - base);
-    double* M = Mw + (size_t)(ib - 1) * msz;
-
-    // ---- M of this block into LDS (identity for the first column)
-    for (int e0 = 0; e0 < ldr * ldr; e0 += 16 * kGenThreads) {
-      double tmp[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int e = e0 + tid + q * kGenThreads;
-        tmp[q] = (ic > 1 && e < ldr * ldr) ? M[e] : (e % ldr == e / ldr ? 1.0 : 0.0);
-      }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int e = e0 + tid + q * kGenThreads;
-        if (e < ldr * ldr) Ml[(e / ldr) * ldl + e % ldr] = tmp[q];
-      }
-    }
-    __syncthreads();
-
-    // ---- deferred left rotations of the sweep's earlier columns:
-    //      A(:, col) and B(:, row0+r-1), local rows 1..L of block ib.
-    //      Thread t: output rows i = t%128 (+128), half h = t/128 of the
-    //      reduction; its loads of M are all in flight together.
-    if (ic > 1 && L > 0) {
-      const bool doB = row0 + r - 1 <= n;
-      for (int t = tid; t < L; t += kGenThreads) {
-        xa[t] = A[PHT_IDX(base + t, col - 1, lda)];
-        xb[t] = doB ? B[PHT_IDX(base + t, row0 + r - 2, ldb)] : 0.0;
-      }
-      __syncthreads();
-      const int h = tid >> 7, il = tid & 127;
-      const int jp = (L + kParts - 1) / kParts, j0 = h * jp, j1 = imin(L, j0 + jp);
-      for (int i0 = 0; i0 < L; i0 += 128) {
-        const int i = i0 + il;
-        double ya = 0.0, yb = 0.0;
-        if (i < L) {
-          for (int j = j0; j < j1; ++j) {
-            const double m = Ml[j * ldl + i];
-            ya += m * xa[j];
-            yb += m * xb[j];
-          }
-        }
-        red[0][h][il] = ya;
-        red[1][h][il] = yb;
-        __syncthreads();
-        if (h == 0 && i < L) {
-          double sa = 0.0, sb = 0.0;
-#pragma unroll
-          for (int q = 0; q < kParts; ++q) {
-            sa += red[0][q][il];
-            sb += red[1][q][il];
-          }
-          A[PHT_IDX(base + i, col - 1, lda)] = sa;
-          if (doB) B[PHT_IDX(base + i, row0 + r - 2, ldb)] = sb;
-        }
-        __syncthreads();
-      }
-    }
-
-    // the reference leaves the column here when nb < 2 (rht2ht.f90:827,911);
-    // later blocks still owe this column their deferred rotations
-    if (nb >= 2) {
-      const int istart = imax(s + 1, s + 1 + (ib - 1 - nq + ic) * r);
-      const int R0 = row0 - 1, C0 = col - 1;  // 0-based anchors
-
-      // ---- left chain: zero A(row0+1:row0+nb-1, col) (rht2ht.f90:830-835,
-      //      914-919).  The chain's partial results are signed suffix norms of
-      //      the column, so every rotation follows from a suffix sum of
-      //      squares (wave 0, lane = row).
-      if (wave == 0) {
-        const double x = (lane < nb) ? A[PHT_IDX(R0 + lane, C0, lda)] : 0.0;
-        const double mx = wave_max(fabs(x));
-        const double xs = (mx > 0.0) ? x / mx : 0.0;
-        double S = xs * xs;  // inclusive suffix sum over lanes >= lane
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const double t = __shfl_down(S, off, 64);
-          if (lane + off < 64) S += t;
-        }
-        const double Snext = __shfl_down(S, 1, 64);  // suffix from lane+1
-        // g: the value the chain leaves at row row0+lane
-        double g;
-        if (lane >= nb - 1 || Snext == 0.0)
-          g = x;
-        else
-          g = (x < 0.0 ? -1.0 : 1.0) * (sqrt(S) * mx);
-        const double f_above = __shfl_up(x, 1, 64);        // x_{lane-1}
-        const double d_above = __shfl_up(fabs(g), 1, 64);  // |g_{lane-1}|
-        if (lane >= 1 && lane < nb) {
-          double c, sn;
-          rot_from_norm(f_above, g, d_above, c, sn);
-          lc[lane] = c;
-          ls[lane] = sn;
-          const size_t ti = tape_idx(ic + lane - 1, ic, ib, ldr, nq);
-          CU[ti] = c;
-          SU[ti] = sn;
-        }
-        if (lane < nb) A[PHT_IDX(R0 + lane, C0, lda)] = (lane == 0) ? g : 0.0;
-      }
-      // B block into LDS (coalesced by rows; all loads in flight together)
-      {
-        constexpr int PER = (RMAX * RMAX + kGenThreads - 1) / kGenThreads;
-        double tmp[PER];
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          const int e = tid + q * kGenThreads;
-          const int k = e % RMAX, j = e / RMAX;
-          tmp[q] = (k < nb && j < nb) ? B[PHT_IDX(R0 + k, R0 + j, ldb)] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          const int e = tid + q * kGenThreads;
-          if (e < RMAX * RMAX) Bb[e % RMAX][e / RMAX] = tmp[q];
-        }
-      }
-      __syncthreads();
-
-      // ---- left rotations on the B block (rht2ht.f90:837-840,921-924):
-      //      threads 0..nb-1, one block column each; meanwhile threads
-      //      64.. fold the same rotations into M (local rows ic-1..ic+nb-2)
-      if (tid < nb)
-        left_chain_lds(&Bb[0][tid], RMAX + 1, nb, lc, ls);
-      else if (tid >= 64 && tid < 64 + ldr)
-        left_chain_lds(&Ml[(tid - 64) * ldl + (ic - 1)], 1, nb, lc, ls);
-      __syncthreads();
-      // the updated M rows go back while wave 0 runs the right chain
-      if (wave > 0) {
-        constexpr int PM = (RMAX * kMaxLdr + kGenThreads - 64 - 1) / (kGenThreads - 64);
-#pragma unroll
-        for (int q = 0; q < PM; ++q) {
-          const int e = (tid - 64) + q * (kGenThreads - 64);
-          const int k = e % RMAX, cm = e / RMAX;
-          if (k < nb && cm < ldr) M[(ic - 1) + k + (size_t)cm * ldr] = Ml[cm * ldl + (ic - 1) + k];
-        }
-      }
-
-      // ---- right chain on the B block (rht2ht.f90:843-851,927-935):
-      //      wave 0, lane = block row; the pivot values are read from lane ig
-      if (wave == 0) right_chain<RMAX>(Bb, nb, lane, rc, rs);
-      __syncthreads();
-      for (int e = tid; e < nb * nb; e += kGenThreads) {
-        const int k = e % nb, j = e / nb;
-        B[PHT_IDX(R0 + k, R0 + j, ldb)] = Bb[k][j];
-      }
-      if (tid >= 1 && tid < nb) {
-        const size_t ti = tape_idx(ic + tid - 1, ic, ib, ldr, nq);
-        CV[ti] = rc[tid];
-        SV[ti] = rs[tid];
-      }
-
-      // ---- right rotations on the window rows (rht2ht.f90:853-867,937-951):
-      //      A rows istart..row0+r+nb-1 and B rows istart..row0-1; with helper
-      //      workgroups the rows above the bulge (istart..row0-1) are theirs
-      {
-        const int aend = imin(n, row0 + r + nb - 1);  // 1-based inclusive
-        const int first = (nhelp > 0) ? row0 : istart;
-        const int nAr = aend - first + 1;
-        const int nBr = (nhelp > 0) ? 0 : row0 - istart;
-        for (int t = tid; t < nAr + nBr; t += kGenThreads) {
-          const bool inA = t < nAr;
-          rotate_row<RMAX>(inA ? A : B, inA ? lda : ldb, first - 1 + (inA ? t : t - nAr), R0,
-                           nb, rc, rs);
-        }
-      }
-    }
+    chase_step<RMAX, kGenThreads>(A, lda, B, ldb, n, r, nq, s, ic, ib, col, row0, CU, SU, CV,
+                                  SV, ldr, Mw, nhelp == 0, Bb, lc, ls, rc, rs, xa, xb,
+                                  &red[0][0][0], Ml);
     // ---- publish the step
     __threadfence();
     __syncthreads();

@@ -1,0 +1,123 @@
+"""Batched persistent path (one launch for many pencils): every pencil is
+reduced to Hessenberg-triangular form with backward errors within 10 n u and
+exact structural zeros, matches the oracle's reduction with the same
+(r, p, nq) after sign normalization, matches the single-pencil GPU path, and
+a HIP-graph replay reproduces the eager launch bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+U = 2.0 ** -53
+
+
+def _host(bt, i):
+    return [bt.view(M, i).cpu().numpy().copy(order="F") for M in (bt.A, bt.B, bt.Q, bt.Z)]
+
+
+def _rel(X, Y, s):
+    return float(np.linalg.norm(X - Y) / s)
+
+
+CASES = [(3, (8, 2, 4)), (17, (8, 2, 4)), (32, (8, 4, 8)), (65, (16, 2, 8)),
+         (100, (16, 4, 16)), (150, (8, 8, 8)), (64, (16, 8, 4)), (129, (16, 2, 16))]
+
+
+def test_batched_matches_oracle(oracle_mod):
+    from paraht_amd.batched import Batch, reduce_batch
+    from paraht_amd.pencils import random_pencil
+    O = oracle_mod
+    print("batched vs oracle", [c for c in CASES], flush=True)
+    sizes = [n for n, _ in CASES]
+    pens = [random_pencil(n, seed=700 + n) for n in sizes]
+    bt = Batch(sizes)
+    bt.load([a for a, _ in pens], [b for _, b in pens])
+    desc = bt.descriptors()
+    for i, (_, (r, p, nq)) in enumerate(CASES):
+        desc[i].r, desc[i].p, desc[i].nq = r, p, nq
+    reduce_batch(bt, desc=desc)
+    for i, (n, (r, p, nq)) in enumerate(CASES):
+        A0, B0 = pens[i]
+        H, T, Q, Z = _host(bt, i)
+        ea, eb = O.backward_errors(A0, B0, H, T, Q, Z)
+        assert ea <= 10 * n * U and eb <= 10 * n * U, (n, ea, eb)
+        assert O.below_band(H, 1) == 0.0 and O.below_band(T, 0) == 0.0, n
+        g = O.sign_normalize(H, T, Q, Z)
+        o = O.sign_normalize(*O.reduce(A0, B0, r, p, nq, stage2="givens"))
+        sa, sb = np.linalg.norm(A0), np.linalg.norm(B0)
+        assert _rel(g[0], o[0], sa) <= 1e-9 and _rel(g[1], o[1], sb) <= 1e-9, (n, r, p, nq)
+        assert _rel(g[2], o[2], np.sqrt(n)) <= 1e-9 and _rel(g[3], o[3], np.sqrt(n)) <= 1e-9
+
+
+def test_batched_uneven_sizes_invariants():
+    import torch
+    from paraht_amd.batched import Batch, reduce_batch
+    rng = np.random.default_rng(1302)
+    sizes = [int(x) for x in rng.integers(32, 513, size=192)]
+    print("batched invariants, 192 pencils", flush=True)
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(1302)
+    As, Bs = [], []
+    for n in sizes:
+        As.append(torch.randn(n, n, dtype=torch.float64, device="cuda:0", generator=g))
+        R = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda:0",
+                                        generator=g), mode="r")[1]
+        Bs.append(torch.triu(R))
+    bt = Batch(sizes)
+    bt.load(As, Bs)
+    reduce_batch(bt)
+    worst = 0.0
+    for i, n in enumerate(sizes):
+        H, T, Q, Z = (bt.view(M, i) for M in (bt.A, bt.B, bt.Q, bt.Z))
+        ea = (torch.linalg.norm(Q.T @ As[i] @ Z - H) / torch.linalg.norm(As[i])).item()
+        eb = (torch.linalg.norm(Q.T @ Bs[i] @ Z - T) / torch.linalg.norm(Bs[i])).item()
+        worst = max(worst, ea / (n * U), eb / (n * U))
+        assert int(torch.count_nonzero(torch.tril(H, -2))) == 0, n
+        assert int(torch.count_nonzero(torch.tril(T, -1))) == 0, n
+        eye = torch.eye(n, dtype=torch.float64, device="cuda:0")
+        assert torch.linalg.norm(Q.T @ Q - eye).item() <= 10 * n * U, n
+        assert torch.linalg.norm(Z.T @ Z - eye).item() <= 10 * n * U, n
+    assert worst <= 10.0, worst
+
+
+def test_batched_matches_single_pencil_path():
+    import torch
+    import paraht_amd as ph
+    from paraht_amd.batched import Batch, reduce_batch
+    from paraht_amd.pencils import random_pencil
+    sizes = [96, 200, 333]
+    params = [(8, 4, 8), (16, 4, 8), (16, 2, 16)]
+    pens = [random_pencil(n, seed=900 + n) for n in sizes]
+    bt = Batch(sizes)
+    bt.load([a for a, _ in pens], [b for _, b in pens])
+    desc = bt.descriptors()
+    for i, (r, p, nq) in enumerate(params):
+        desc[i].r, desc[i].p, desc[i].nq = r, p, nq
+    reduce_batch(bt, desc=desc)
+    for i, (n, (r, p, nq)) in enumerate(zip(sizes, params)):
+        A0, B0 = pens[i]
+        cm = lambda M: ph.column_major(torch.from_numpy(np.ascontiguousarray(M)).to("cuda:0"))
+        H, T, Q, Z = ph.reduce(cm(A0), cm(B0), r, p, nq)
+        for X, Y, sc in ((bt.view(bt.A, i), H, np.linalg.norm(A0)),
+                         (bt.view(bt.B, i), T, np.linalg.norm(B0)),
+                         (bt.view(bt.Q, i), Q, np.sqrt(n)), (bt.view(bt.Z, i), Z, np.sqrt(n))):
+            assert torch.linalg.norm(X - Y).item() / sc <= 1e-9, (n, r, p, nq)
+
+
+def test_batched_graph_replay_is_exact():
+    import torch
+    from paraht_amd.batched import Batch, reduce_batch
+    from paraht_amd.pencils import random_pencil
+    sizes = [40, 77, 128, 250, 64, 300]
+    pens = [random_pencil(n, seed=1100 + n) for n in sizes]
+    bt = Batch(sizes)
+    bt.load([a for a, _ in pens], [b for _, b in pens])
+    reduce_batch(bt)
+    ref = [M.clone() for M in (bt.A, bt.B, bt.Q, bt.Z)]
+    desc = bt.descriptors()
+    for rep in range(3):
+        bt.load([a for a, _ in pens], [b for _, b in pens])
+        torch.cuda.synchronize()
+        reduce_batch(bt, desc=desc, graph=True)
+        for X, Y in zip((bt.A, bt.B, bt.Q, bt.Z), ref):
+            assert torch.equal(X, Y), rep
