@@ -141,6 +141,54 @@ def bench_batched(args):
             "max_backward_error_every_64th": worst}
 
 
+def bench_adjoint(args):
+    """Reverse mode: 64 FP64 pencils with n=256 (batched forward, one
+    workgroup per pencil in the adjoint) plus one pencil with n=2048
+    (single-pencil forward, cooperative-grid adjoint), A ~ N(0,1), B = R of
+    the QR of an N(0,1) matrix; loss = sum W*H + V*T + P*Q + S*Z with W, V,
+    P, S ~ N(0,1).  Forward time and forward + backward time (best of 3)."""
+    import torch
+
+    from paraht_amd.adjoint import ht_reduce
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+
+    def pencils(b, n):
+        A = torch.randn(b, n, n, dtype=torch.float64, device=dev, generator=g)
+        R = torch.linalg.qr(torch.randn(b, n, n, dtype=torch.float64, device=dev, generator=g),
+                            mode="r")[1]
+        return A, torch.triu(R)
+
+    parts = [pencils(64, 256), pencils(1, 2048)]
+    parts = [parts[0], (parts[1][0][0], parts[1][1][0])]
+    weights = [[torch.randn(A.shape, dtype=torch.float64, device=dev, generator=g)
+                for _ in range(4)] for A, _ in parts]
+
+    def run(backward):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for (A0, B0), (W, V, P, S) in zip(parts, weights):
+            A = A0.detach().requires_grad_(backward)
+            B = B0.detach().requires_grad_(backward)
+            H, T, Q, Z = ht_reduce(A, B)
+            if backward:
+                loss = ((torch.triu(W, -1) * H).sum() + (torch.triu(V) * T).sum()
+                        + (P * Q).sum() + (S * Z).sum())
+                torch.autograd.grad(loss, (A, B))
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    run(False)
+    run(True)
+    t_f = min(run(False) for _ in range(3))
+    t_fb = min(run(True) for _ in range(3))
+    print(f"adjoint: forward {t_f:.3f} s, forward+backward {t_fb:.3f} s",
+          file=sys.stderr, flush=True)
+    return {"workload": "64 pencils n=256 + 1 pencil n=2048; loss sum W*H+V*T+P*Q+S*Z",
+            "forward_s": t_f, "forward_backward_s": t_fb, "ratio": t_fb / t_f}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +205,8 @@ def main():
                     help="skip the batched section (2048 pencils, n uniform in [32, 512])")
     ap.add_argument("--batched-loop-sample", type=int, default=48,
                     help="pencils timed through the single-pencil path for the loop baseline")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the forward + backward measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="before the warmup, launch a plain elementwise copy of "
                          "1 GiB three times (the reference kernel for scaling "
@@ -272,6 +322,8 @@ def main():
                                "algorithmic_per_launch": (fl if bound == "mfma" else by) / cnt}
     if not args.no_batched:
         out["batched"] = bench_batched(args)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

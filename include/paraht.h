@@ -134,6 +134,22 @@ int paraht_batched_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils,
  * caller passes zeros (the size-bucket table). */
 int paraht_batched_defaults(int n, int* r, int* p, int* nq);
 
+/* ---- Reverse-mode adjoint of the reduction ----
+ * No counterpart in the reference.  For reduced pencils (H, T) with
+ * cotangents folded into g_Q, g_Z (strictly lower parts of G - G^T, with
+ * G_Q = Q^T Qbar - Hbar H^T - Tbar T^T and G_Z = Z^T Zbar + H^T Hbar +
+ * T^T Tbar, column 0 of g_Q zero), runs the transposed recurrence of the
+ * structure constraints on dH, dT and writes X_A (rows >= j+2 of each column
+ * j) and X_B (rows >= j+1); then Abar = Q (hess(Hbar) + X_A) Z^T and
+ * Bbar = Q (triu(Tbar) + X_B) Z^T.  Tinv is T^{-1}.  All arrays are device
+ * pools holding each pencil's n x n column-major matrix (ld = n) at its
+ * offset; dGQ is overwritten (workspace); dXA, dXB must be zero on entry.
+ * One pencil of order >= 512 runs over a cooperative grid, otherwise one
+ * workgroup per pencil.  Requires H(j+1, j) != 0 and T nonsingular. */
+int paraht_adjoint_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils,
+                       const double* dH, const double* dT, const double* dTinv, double* dGQ,
+                       const double* dGZ, double* dXA, double* dXB);
+
 /* Library version (major*10000 + minor*100 + patch). */
 int paraht_version(void);
 /* Human-readable text of a status code. */

@@ -31,7 +31,7 @@ from ._lib import PARAHT_STAGE2_GIVENS, PARAHT_TIME_COMPONENTS, ParahtError
 __all__ = [
     "ab2rht_simple", "rht2ht_householder", "rht2ht_givens",
     "rht2ht_householder_unblocked", "reduce", "Context", "ParahtError",
-    "build", "column_major", "batched",
+    "build", "column_major", "batched", "adjoint",
 ]
 
 build = _lib.build
@@ -126,6 +126,13 @@ def _ctx_for(t) -> Context:
     return _DEFAULT_CTX[dev]
 
 
+def _torch_ready(t) -> None:
+    """Waits for torch's queued work on t's device: the library runs on its
+    context's own non-blocking stream, which is not ordered with torch's."""
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
+
+
 def _dispatch(host_name, dev_name, what, n, mats, ints, flags, ctx=None):
     L = _lib.lib()
     names = ("A", "B", "Q", "Z")
@@ -137,6 +144,7 @@ def _dispatch(host_name, dev_name, what, n, mats, ints, flags, ctx=None):
                 raise ValueError(f"ld{nm.lower()}={ld} does not match the tensor's stride {sld}")
             lds.append(sld)
         c = ctx or _ctx_for(mats[0][0])
+        _torch_ready(mats[0][0])
         args = [c._h, n]
         for (m, _), ld in zip(mats, lds):
             args += [ctypes.c_void_p(m.data_ptr()), ld]
@@ -209,3 +217,4 @@ def reduce(A, B, r, p, nq, Q=None, Z=None, time_components=False, ctx=None):
 
 
 from . import batched  # noqa: E402  (batched reduction of many small pencils)
+from . import adjoint  # noqa: E402  (reverse mode: ht_backward, ht_reduce)

@@ -48,6 +48,8 @@ SIGNATURES = {
     "paraht_batched_dev": ([_P, _I, ctypes.POINTER(Pencil), _DP, _DP, _DP, _DP, _I], _I),
     "paraht_batched_defaults": ([_I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                  ctypes.POINTER(_I)], _I),
+    "paraht_adjoint_dev": ([_P, _I, ctypes.POINTER(Pencil), _DP, _DP, _DP, _DP, _DP, _DP, _DP],
+                           _I),
     "paraht_version": ([], _I),
     "paraht_status_string": ([_I], ctypes.c_char_p),
 }

@@ -77,8 +77,9 @@ class Batch:
 def reduce_batch(batch: Batch, params=None, ctx=None, graph=False, desc=None):
     """Reduces every pencil of `batch` in place (A -> H, B -> T, Q, Z
     accumulated) with one persistent launch."""
-    from . import Context, _ctx_for
+    from . import _ctx_for, _torch_ready
     ctx = ctx or _ctx_for(batch.A)
+    _torch_ready(batch.A)
     d = desc if desc is not None else batch.descriptors(params)
     flags = PARAHT_BATCH_GRAPH if graph else 0
     _lib.check(_lib.lib().paraht_batched_dev(

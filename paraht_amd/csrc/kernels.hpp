@@ -104,4 +104,22 @@ hipError_t batched_launch(hipStream_t st, const BatchPencil* dpencils, int count
                           double* Ap, double* Bp, double* Qp, double* Zp, double* ws,
                           size_t ws_per_wg, int grid);
 
+// ---- adjoint (reverse mode) of the reduction ----
+// Dynamic LDS of the workgroup-per-pencil adjoint kernel for pencils of
+// order <= nmax, and the most it accepts (4 vectors of n doubles: n <= 4992).
+size_t adjoint_wg_lds(int nmax);
+size_t adjoint_wg_max_lds();
+// Transposed recurrence for a queue of pencils (largest first), one
+// workgroup each; matrices in pools at the pencils' offsets (ld = n).
+hipError_t adjoint_batched(hipStream_t st, const BatchPencil* dpencils, int count, int nmax,
+                           int* dnext, const double* H, const double* T, const double* Tinv,
+                           double* GQ, const double* GZ, double* XA, double* XB, int grid);
+// The same for one pencil over a cooperative grid (vecs: 4n doubles; ints:
+// barrier counter, generation, error flag).
+hipError_t adjoint_grid(hipStream_t st, int n, const double* H, const double* T,
+                        const double* Tinv, double* GQ, const double* GZ, double* XA,
+                        double* XB, double* vecs, int* ints, int grid);
+// Largest co-resident grid of the cooperative adjoint kernel (0 on failure).
+int adjoint_grid_max_blocks(int device);
+
 }  // namespace pht

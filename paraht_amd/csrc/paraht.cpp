@@ -669,6 +669,63 @@ int batched(paraht_ctx* c, int count, const paraht_pencil* pencils, double* dA, 
 }  // namespace
 
 // ------------------------------------------------------------------------
+// Adjoint
+// ------------------------------------------------------------------------
+namespace {
+
+constexpr int kAdjGridMinN = 512;  // one pencil this large uses the whole grid
+
+int adjoint(paraht_ctx* c, int count, const paraht_pencil* pencils, const double* H,
+            const double* T, const double* Tinv, double* GQ, const double* GZ, double* XA,
+            double* XB) {
+  std::vector<pht::BatchPencil> q;
+  q.reserve(count);
+  for (int i = 0; i < count; ++i) {
+    const paraht_pencil& pc = pencils[i];
+    if (pc.n < 0 || pc.offset < 0) return -3;
+    if (pc.n >= 2) q.push_back({pc.offset, pc.n, 0, 0, 0});
+  }
+  if (q.empty()) return 0;
+  std::stable_sort(q.begin(), q.end(), [](const pht::BatchPencil& a, const pht::BatchPencil& b) {
+    return a.n > b.n;
+  });
+  hipStream_t st = c->stream;
+  const int nmax = q[0].n;
+  if (q.size() == 1 && nmax >= kAdjGridMinN) {
+    const int maxb = pht::adjoint_grid_max_blocks(c->device);
+    const int grid = std::min(c->cu_count, maxb);
+    if (grid <= 0) return PARAHT_ERR_UNSUPPORTED;
+    if (int e = ensure_ws(c, (size_t)4 * nmax * sizeof(double) + 256)) return e;
+    double* vecs = (double*)c->ws;
+    int* ints = (int*)(vecs + 4 * (size_t)nmax);  // in the workspace's 256 spare bytes
+    const size_t o = (size_t)q[0].offset;
+    CK(pht::adjoint_grid(st, nmax, H + o, T + o, Tinv + o, GQ + o, GZ + o, XA + o, XB + o, vecs,
+                         ints, grid));
+    int err = 0;
+    CK(hipMemcpyAsync(&err, ints + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (err) {
+      if (verbose()) fprintf(stderr, "paraht: adjoint grid barrier exceeded its spin limit\n");
+      return PARAHT_ERR_HIP;
+    }
+    return 0;
+  }
+  if (pht::adjoint_wg_lds(nmax) > pht::adjoint_wg_max_lds()) return PARAHT_ERR_UNSUPPORTED;
+  const int nq_ = (int)q.size();
+  const int grid = std::min(nq_, c->cu_count);
+  const size_t pbytes = ((nq_ * sizeof(pht::BatchPencil) + 255) / 256) * 256;
+  if (int e = ensure_ws(c, pbytes + 256)) return e;
+  auto* dP = (pht::BatchPencil*)c->ws;
+  int* dnext = (int*)((char*)c->ws + pbytes);
+  CK(hipMemcpyAsync(dP, q.data(), nq_ * sizeof(pht::BatchPencil), hipMemcpyHostToDevice, st));
+  CK(pht::adjoint_batched(st, dP, nq_, nmax, dnext, H, T, Tinv, GQ, GZ, XA, XB, grid));
+  CK(hipStreamSynchronize(st));
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
 // C-ABI
 // ------------------------------------------------------------------------
 extern "C" {
@@ -823,6 +880,24 @@ int paraht_batched_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils,
   if (!dZ) return -7;
   CK(hipSetDevice(ctx->device));
   return batched(ctx, count, pencils, dA, dB, dQ, dZ, flags);
+}
+
+int paraht_adjoint_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils,
+                       const double* dH, const double* dT, const double* dTinv, double* dGQ,
+                       const double* dGZ, double* dXA, double* dXB) {
+  if (!ctx) return -1;
+  if (count < 0) return -2;
+  if (count == 0) return 0;
+  if (!pencils) return -3;
+  if (!dH) return -4;
+  if (!dT) return -5;
+  if (!dTinv) return -6;
+  if (!dGQ) return -7;
+  if (!dGZ) return -8;
+  if (!dXA) return -9;
+  if (!dXB) return -10;
+  CK(hipSetDevice(ctx->device));
+  return adjoint(ctx, count, pencils, dH, dT, dTinv, dGQ, dGZ, dXA, dXB);
 }
 
 int paraht_batched_defaults(int n, int* r, int* p, int* nq) {
