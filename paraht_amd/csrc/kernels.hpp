@@ -21,10 +21,11 @@ enum Side { kLeftT = 0, kRightN = 1 };
 
 // Applies blocks[0..nblocks) in order: C <- H^T C (kLeftT) or C <- C H
 // (kRightN), on FP64 MFMA tiles.  [other_begin, other_end) bounds the other
-// dimension over all blocks (it sets the grid).  kmax = max k over blocks.
+// dimension over all blocks (it sets the grid).  kmax = max k and max_len =
+// max len over the blocks (len <= 128 and k <= 64 take the LDS-staged kernel).
 hipError_t apply_blocks(hipStream_t st, Side side, double* C, int ldc,
                         const double* V, int ldv, const double* T, int ldt,
-                        const ApplyBlock* dblocks, int nblocks, int kmax,
+                        const ApplyBlock* dblocks, int nblocks, int kmax, int max_len,
                         int other_begin, int other_end);
 
 // ---- stage 1 (ab2rht_simple) ----

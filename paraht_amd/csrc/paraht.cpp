@@ -129,39 +129,43 @@ int prof_reserve(paraht_ctx* c, size_t need) {
   return 0;
 }
 
-int prof_event(paraht_ctx* c, int* idx) {
+int prof_event(paraht_ctx* c, int* idx, hipStream_t st) {
   *idx = (int)c->pool_used++;
-  CK(hipEventRecord(c->pool[*idx], c->stream));
+  CK(hipEventRecord(c->pool[*idx], st));
   return 0;
 }
 
 struct Prof {
   paraht_ctx* c;
+  hipStream_t st;
   int kid, e0 = -1;
   double flops, bytes;
-  Prof(paraht_ctx* cx, int k, double f, double b) : c(cx), kid(k), flops(f), bytes(b) {}
+  Prof(paraht_ctx* cx, hipStream_t s, int k, double f, double b)
+      : c(cx), st(s), kid(k), flops(f), bytes(b) {}
   int begin() {
     if (!c->profiling) return 0;
     if (int e = prof_reserve(c, 2)) return e;
-    return prof_event(c, &e0);
+    return prof_event(c, &e0, st);
   }
   int end() {
     if (!c->profiling) return 0;
     int e1 = -1;
-    if (int e = prof_event(c, &e1)) return e;
+    if (int e = prof_event(c, &e1, st)) return e;
     c->recs.push_back({kid, e0, e1, flops, bytes});
     return 0;
   }
 };
 
-// Runs `launch` between the profiling events of kernel class `kid`.
-#define PROF(kid, flops, bytes, launch)                   \
+// Runs `launch` (enqueued on stream `strm`) between the profiling events of
+// kernel class `kid`.
+#define PROF_ON(strm, kid, flops, bytes, launch)          \
   do {                                                    \
-    Prof _p(c, (kid), (flops), (bytes));                  \
+    Prof _p(c, (strm), (kid), (flops), (bytes));          \
     if (int _e = _p.begin()) return _e;                   \
     CK(launch);                                           \
     if (int _e = _p.end()) return _e;                     \
   } while (0)
+#define PROF(kid, flops, bytes, launch) PROF_ON(c->stream, kid, flops, bytes, launch)
 
 // algorithmic flops of a list of compact-WY blocks: 4*len*k*width each
 double wy_flops(const std::vector<ApplyBlock>& v, size_t first, int cnt) {
@@ -337,13 +341,13 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     const int m1 = tm.mark();
     PROF(PARAHT_K_WY_APPLY, wy_flops(BL, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kLeftT, B, ldb, VL, P, TL, nb, dBL + pi.first,
-                           pi.nblocks, nb, pi.i0top, n));
+                           pi.nblocks, nb, P, pi.i0top, n));
     PROF(PARAHT_K_WY_APPLY, wy_flops(AL, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kLeftT, A, lda, VL, P, TL, nb, dAL + pi.first,
-                           pi.nblocks, nb, pi.jb0 + nb, n));
+                           pi.nblocks, nb, P, pi.jb0 + nb, n));
     PROF(PARAHT_K_WY_APPLY, wy_flops(QL, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kRightN, Q, ldq, VL, P, TL, nb, dQL + pi.first,
-                           pi.nblocks, nb, 0, n));
+                           pi.nblocks, nb, P, 0, n));
     const int m2 = tm.mark();
     for (int bl = 0; bl < pi.nblocks; ++bl) {
       const ApplyBlock& b = BR[pi.first + bl];
@@ -358,17 +362,17 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
                             TR + (size_t)bl * nb * nb, nb, scratch));
       PROF(PARAHT_K_WY_APPLY, wy_flops(BR, pi.first + bl, 1), 0.0,
            pht::apply_blocks(st, pht::kRightN, B, ldb, VR, P, TR, nb,
-                             dBR + pi.first + bl, 1, b.k, 0, b.start + b.len));
+                             dBR + pi.first + bl, 1, b.k, P, 0, b.start + b.len));
       PROF(PARAHT_K_OTHER, 0.0, 0.0,
            pht::zero_below(st, B, ldb, b.start, b.len, b.len > nb ? nb : b.len));
     }
     const int m3 = tm.mark();
     PROF(PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
-                           pi.nblocks, nb, 0, n));
+                           pi.nblocks, nb, P, 0, n));
     PROF(PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kRightN, Z, ldz, VR, P, TR, nb, dRR + pi.first,
-                           pi.nblocks, nb, 0, n));
+                           pi.nblocks, nb, P, 0, n));
     const int m4 = tm.mark();
     marks.insert(marks.end(), {m0, m1, m2, m3, m4});
   }
