@@ -78,11 +78,13 @@ hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
                        const S2Params& P, double* CU, double* SU, double* CV,
                        double* SV);
 // Remaining right rotations (rows of A,B above the generation window, all
-// rows of Z) and left rotations seen from the right (all rows of Q).
+// rows of Z) and left rotations seen from the right (all rows of Q); part
+// selects the A/B rows, the Q/Z rows or both.
+enum RowsPart { kRowsAll = 0, kRowsAB = 1, kRowsQZ = 2 };
 hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
                          int ldb, double* Q, int ldq, double* Z, int ldz,
                          const S2Params& P, const double* CU, const double* SU,
-                         const double* CV, const double* SV);
+                         const double* CV, const double* SV, RowsPart part);
 // Remaining left rotations (columns of A,B right of the generation window).
 hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
                          int ldb, const S2Params& P, const double* CU,

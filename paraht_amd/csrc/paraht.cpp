@@ -34,6 +34,11 @@ struct paraht_ctx {
   int cu_count = 256;  // compute units of the device
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
+  // second stream for work off the critical path (stage 2's Q/Z updates),
+  // joined back into `stream` before a call returns
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_tape = nullptr;   // a sweep's tape is complete (stream -> aux)
+  hipEvent_t ev_free[2] = {nullptr, nullptr};  // aux is done with tape buffer b
   void* ws = nullptr;
   size_t ws_bytes = 0;
   double times[8] = {0};
@@ -102,6 +107,7 @@ int check_common(int n, int lda, int ldb, int ldq, int ldz) {
 int prof_flush(paraht_ctx* c) {
   if (c->recs.empty()) return 0;
   CK(hipStreamSynchronize(c->stream));
+  if (c->aux) CK(hipStreamSynchronize(c->aux));
   for (const KernelRec& r : c->recs) {
     float t = 0.f;
     CK(hipEventElapsedTime(&t, c->pool[r.e0], c->pool[r.e1]));
@@ -166,6 +172,16 @@ struct Prof {
     if (int _e = _p.end()) return _e;                     \
   } while (0)
 #define PROF(kid, flops, bytes, launch) PROF_ON(c->stream, kid, flops, bytes, launch)
+
+// Creates the context's second stream (default priority) and its events.
+int ensure_aux(paraht_ctx* c) {
+  if (c->aux) return 0;
+  CK(hipEventCreateWithFlags(&c->ev_tape, hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->ev_free[0], hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->ev_free[1], hipEventDisableTiming));
+  CK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+  return 0;
+}
 
 // algorithmic flops of a list of compact-WY blocks: 4*len*k*width each
 double wy_flops(const std::vector<ApplyBlock>& v, size_t first, int cnt) {
@@ -417,16 +433,23 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   int nbl = (n - 1) / r + ((n - 1) % r != 0);
   const size_t tsz = (size_t)ldr * nq * nbl;
   const size_t msz = (size_t)ldr * ldr * nbl;
-  if (int e = ensure_ws(c, (4 * tsz + msz) * sizeof(double) + pht::s2_counter_ints(nq) * sizeof(int)))
+  // two tape buffers: the Q/Z updates of sweep k run on the second stream
+  // while sweep k+1 is generated into the other buffer
+  if (int e = ensure_ws(c, (8 * tsz + msz) * sizeof(double) + pht::s2_counter_ints(nq) * sizeof(int)))
     return e;
-  double* CU = (double*)c->ws;
-  double* SU = CU + tsz;
-  double* CV = SU + tsz;
-  double* SV = CV + tsz;
+  if (int e = ensure_aux(c)) return e;
+  double* tape[2] = {(double*)c->ws, (double*)c->ws + 4 * tsz};
+  double* Mws = tape[1] + 4 * tsz;
   const bool tc = (flags & PARAHT_TIME_COMPONENTS) != 0;
   Timer tm(c, tc);
   std::vector<int> marks;
-  for (int s = 1; s <= n - 2; s += nq) {
+  int buf = 0;
+  for (int s = 1, k = 0; s <= n - 2; s += nq, ++k) {
+    buf = k & 1;
+    double* CU = tape[buf];
+    double* SU = CU + tsz;
+    double* CV = SU + tsz;
+    double* SV = CV + tsz;
     pht::S2Params P;
     P.n = n;
     P.r = r;
@@ -434,7 +457,7 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     P.jblock = s;
     P.nblocks = (n - s) / r + ((n - s) % r != 0);
     P.ldr = ldr;
-    P.M = SV + tsz;
+    P.M = Mws;
     P.progress = (int*)(P.M + msz);
     P.hprog = P.progress + nq;
     // helpers need every workgroup resident and nq <= r+1 (a column's steps
@@ -443,6 +466,8 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     if (P.nhelp < 0) P.nhelp = 0;
     P.errflag = P.progress + (size_t)nq * (1 + pht::kS2MaxHelpers);
     const int m0 = tm.mark();
+    // the buffer's previous tape (sweep k-2) must be consumed by its Q/Z update
+    if (k >= 2) CK(hipStreamWaitEvent(st, c->ev_free[buf], 0));
     PROF(PARAHT_K_OTHER, 0.0, 0.0, pht::s2_prepare(st, P, CU, SU, CV, SV));
     PROF(PARAHT_K_S2_GEN, s2_gen_flops(n, r, nq, s), 0.0,
          pht::s2_generate(st, A, lda, B, ldb, P, CU, SU, CV, SV));
@@ -451,16 +476,25 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     // Q and Z, and of A and B above the window, is read and written once
     const double w = (double)(n - s);
     const double rows_ab = std::min<double>(n, s + (double)(P.nblocks - 1) * r);
-    PROF(PARAHT_K_S2_ROWS, 0.0, 16.0 * w * (2.0 * n + 2.0 * rows_ab),
-         pht::s2_apply_rows(st, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV));
+    CK(hipEventRecord(c->ev_tape, st));
+    CK(hipStreamWaitEvent(c->aux, c->ev_tape, 0));
+    PROF_ON(c->aux, PARAHT_K_S2_ROWS, 0.0, 16.0 * w * 2.0 * n,
+            pht::s2_apply_rows(c->aux, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
+                               pht::kRowsQZ));
+    CK(hipEventRecord(c->ev_free[buf], c->aux));
+    PROF(PARAHT_K_S2_ROWS, 0.0, 16.0 * w * 2.0 * rows_ab,
+         pht::s2_apply_rows(st, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
+                            pht::kRowsAB));
     PROF(PARAHT_K_S2_COLS, 0.0, s2_cols_bytes(n, r, nq, s, P.nblocks),
          pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU));
     const int m2 = tm.mark();
     marks.insert(marks.end(), {m0, m1, m2});
   }
+  // join: the last Q/Z update (aux runs its updates in order)
+  CK(hipStreamWaitEvent(st, c->ev_free[buf], 0));
   {
     int err = 0;
-    CK(hipMemcpyAsync(&err, (int*)(SV + tsz + msz) + (size_t)nq * (1 + pht::kS2MaxHelpers),
+    CK(hipMemcpyAsync(&err, (int*)(Mws + msz) + (size_t)nq * (1 + pht::kS2MaxHelpers),
                        sizeof(int), hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     if (err) {
@@ -763,10 +797,15 @@ int paraht_ctx_destroy(paraht_ctx* ctx) {
   if (!ctx) return -1;
   (void)hipSetDevice(ctx->device);
   if (ctx->own) (void)hipStreamSynchronize(ctx->own);
+  if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
   if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
   if (ctx->ws) (void)hipFree(ctx->ws);
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->own) (void)hipStreamDestroy(ctx->own);
+  if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+  if (ctx->ev_tape) (void)hipEventDestroy(ctx->ev_tape);
+  for (hipEvent_t e : ctx->ev_free)
+    if (e) (void)hipEventDestroy(e);
   delete ctx;
   return 0;
 }

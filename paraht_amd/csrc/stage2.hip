@@ -200,9 +200,9 @@ __global__ __launch_bounds__(kTile) void k_s2_rows(
     double* __restrict__ Q, int ldq, double* __restrict__ Z, int ldz, int n,
     int r, int nq, int s, int nblocks, const double* __restrict__ CU,
     const double* __restrict__ SU, const double* __restrict__ CV,
-    const double* __restrict__ SV, int ldr) {
+    const double* __restrict__ SV, int ldr, int which0) {
   extern __shared__ double win[];  // [ldr][kTile]
-  const int which = blockIdx.y;
+  const int which = which0 + blockIdx.y;
   const int t = threadIdx.x;
   const int i = blockIdx.x * kTile + t;  // 0-based row
   double* M;
@@ -577,7 +577,7 @@ hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
 hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
                          int ldb, double* Q, int ldq, double* Z, int ldz,
                          const S2Params& P, const double* CU, const double* SU,
-                         const double* CV, const double* SV) {
+                         const double* CV, const double* SV, RowsPart part) {
   const size_t lds = (size_t)P.ldr * kTile * sizeof(double);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   static bool attr = false;
@@ -591,20 +591,22 @@ hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
 #define PHT_ROWS(RR, NN)                                                        \
   if (P.r == RR && P.nq == NN) {                                                \
     const dim3 g((P.n + kRowsThreads - 1) / kRowsThreads, 2);                   \
-    hipLaunchKernelGGL((k_s2_rows_v<RR, NN, true>), g, dim3(kRowsThreads), 0,   \
-                       st, A, lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,       \
-                       P.nblocks, CU, SU, CV, SV);                              \
-    hipLaunchKernelGGL((k_s2_rows_v<RR, NN, false>), g, dim3(kRowsThreads), 0,  \
-                       st, A, lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,       \
-                       P.nblocks, CU, SU, CV, SV);                              \
+    if (part != kRowsQZ)                                                        \
+      hipLaunchKernelGGL((k_s2_rows_v<RR, NN, true>), g, dim3(kRowsThreads), 0, \
+                         st, A, lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,     \
+                         P.nblocks, CU, SU, CV, SV);                            \
+    if (part != kRowsAB)                                                        \
+      hipLaunchKernelGGL((k_s2_rows_v<RR, NN, false>), g, dim3(kRowsThreads), 0, \
+                         st, A, lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,     \
+                         P.nblocks, CU, SU, CV, SV);                            \
     return hipGetLastError();                                                   \
   }
   PHT_S2_PAIRS(PHT_ROWS)
 #undef PHT_ROWS
-  dim3 grid((P.n + kTile - 1) / kTile, 4), block(kTile);
+  dim3 grid((P.n + kTile - 1) / kTile, part == kRowsAll ? 4 : 2), block(kTile);
   hipLaunchKernelGGL(k_s2_rows, grid, block, lds, st, A, lda, B, ldb, Q, ldq, Z,
                      ldz, P.n, P.r, P.nq, P.jblock, P.nblocks, CU, SU, CV, SV,
-                     P.ldr);
+                     P.ldr, part == kRowsQZ ? 2 : 0);
   return hipGetLastError();
 }
 
