@@ -31,6 +31,15 @@ namespace pht {
 namespace {
 constexpr int kGenThreads = 512;
 constexpr int kTile = 128;
+// far updates in wavefront form (PARAHT_S2_WAVEFRONT=0: the sequential form)
+bool s2_wavefront() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("PARAHT_S2_WAVEFRONT");
+    v = (e && *e == '0') ? 0 : 1;
+  }
+  return v == 1;
+}
 }  // namespace
 using namespace s2;
 
@@ -510,6 +519,244 @@ __global__ __launch_bounds__(64) void k_s2_cols_v(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront form of the far updates.  A block's NQ rotation sequences act on
+// a window of R+NQ-1 entries; sequence icm applies ig = R-1..1 to the pairs
+// (icm+ig-1, icm+ig).  Rotation (icm+1, ig) only has to follow (icm, ig)
+// (every other rotation of sequence icm that shares one of its entries comes
+// earlier in that sequence), so rotation (icm, ig) can run at step
+// tau = (R-1-ig) + icm, and the rotations of one step act on disjoint pairs:
+// R+NQ-2 dependent steps instead of NQ (R-1).  Sequences below icm0 are
+// skipped (identity rotations, exact).
+// First and last step at which the wavefront touches window entry e:
+// rotation (icm, ig) touches entries icm+ig-1 and icm+ig at step
+// R-1-ig+icm, so as the upper entry of a pair e is touched for icm in
+// [max(0, e-R+1), min(NQ-1, e-1)] at steps R-1-e+2 icm, as the lower entry
+// for icm in [max(0, e+2-R), min(NQ-1, e)] at steps R-2-e+2 icm.
+constexpr int wf_max(int a, int b) { return a > b ? a : b; }
+constexpr int wf_min(int a, int b) { return a < b ? a : b; }
+constexpr int wf_first(int R, int NQ, int e) {
+  int f = 1 << 30;
+  if (wf_max(0, e - R + 1) <= wf_min(NQ - 1, e - 1)) f = R - 1 - e + 2 * wf_max(0, e - R + 1);
+  if (wf_max(0, e + 2 - R) <= wf_min(NQ - 1, e))
+    f = wf_min(f, R - 2 - e + 2 * wf_max(0, e + 2 - R));
+  return f;
+}
+constexpr int wf_last(int R, int NQ, int e) {
+  int l = -1;
+  if (wf_max(0, e - R + 1) <= wf_min(NQ - 1, e - 1)) l = R - 1 - e + 2 * wf_min(NQ - 1, e - 1);
+  if (wf_max(0, e + 2 - R) <= wf_min(NQ - 1, e))
+    l = wf_max(l, R - 2 - e + 2 * wf_min(NQ - 1, e));
+  return l;
+}
+
+// Window entries whose first touch is step TAU+1 are read from the LDS tile
+// (one step ahead), entries whose last touch was step TAU are written back.
+template <int R, int NQ, int TAU, int E>
+__device__ __forceinline__ void wf_stream(double (&w)[R + NQ - 1], double (*tile)[65], int t) {
+  constexpr int NW = R + NQ - 1;
+  if constexpr (wf_first(R, NQ, E) == TAU + 1) w[E] = tile[E][t];
+  if constexpr (wf_last(R, NQ, E) == TAU) tile[E][t] = w[E];
+  if constexpr (E + 1 < NW) wf_stream<R, NQ, TAU, E + 1>(w, tile, t);
+}
+
+// Coefficients of step TAU (sequence ICM's rotation ig = R-1-TAU+ICM) into
+// cs[ICM]; entries of sequences without a rotation at that step are unused.
+template <int R, int NQ, int TAU, int ICM>
+__device__ __forceinline__ void wf_coeffs(double2 (&cs)[NQ], const TapeBlock<R, NQ>& tb) {
+  constexpr int ig = R - 1 - TAU + ICM;
+  if constexpr (ig >= 1 && ig <= R - 1) cs[ICM] = tb.cs[ICM][ig];
+  if constexpr (ICM + 1 < NQ) wf_coeffs<R, NQ, TAU, ICM + 1>(cs, tb);
+}
+
+// The rotations of step TAU: (icm, ig = R-1-TAU+icm) for each sequence.
+template <int R, int NQ, int TAU, int ICM>
+__device__ __forceinline__ void wf_rotations(double (&w)[R + NQ - 1], const double2 (&cs)[NQ],
+                                             int icm0) {
+  constexpr int ig = R - 1 - TAU + ICM;
+  if constexpr (ig >= 1 && ig <= R - 1) {
+    const bool on = ICM >= icm0;
+    const double c = on ? cs[ICM].x : 1.0, sn = on ? cs[ICM].y : 0.0;
+    const double x = w[ICM + ig], y = w[ICM + ig - 1];
+    w[ICM + ig] = c * x + sn * y;
+    w[ICM + ig - 1] = c * y - sn * x;
+  }
+  if constexpr (ICM + 1 < NQ) wf_rotations<R, NQ, TAU, ICM + 1>(w, cs, icm0);
+}
+
+// Step TAU with the next step's coefficients and window entries read ahead.
+template <int R, int NQ, int TAU>
+__device__ __forceinline__ void wf_steps(double (&w)[R + NQ - 1], double (*tile)[65], int t,
+                                         const TapeBlock<R, NQ>& tb, int icm0,
+                                         const double2 (&cs)[NQ]) {
+  double2 nx[NQ];
+  if constexpr (TAU + 1 < R + NQ - 2) wf_coeffs<R, NQ, TAU + 1, 0>(nx, tb);
+  wf_rotations<R, NQ, TAU, 0>(w, cs, icm0);
+  wf_stream<R, NQ, TAU, 0>(w, tile, t);
+  // one step's coefficients and streamed entries in flight at a time
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (TAU + 1 < R + NQ - 2) wf_steps<R, NQ, TAU + 1>(w, tile, t, tb, icm0, nx);
+}
+
+// Applies a block's NQ rotation sequences to column t of the LDS tile
+// (tile[e][t] = window entry e), in wavefront order; only the band of
+// entries the current steps touch is held in registers.
+template <int R, int NQ>
+__device__ __forceinline__ void wavefront_window(double (*tile)[65], int t,
+                                                 const TapeBlock<R, NQ>& tb, int icm0) {
+  double w[R + NQ - 1];
+  double2 cs[NQ];
+  wf_coeffs<R, NQ, 0, 0>(cs, tb);
+  wf_stream<R, NQ, -1, 0>(w, tile, t);  // entries first touched at step 0
+  wf_steps<R, NQ, 0>(w, tile, t, tb, icm0, cs);
+}
+
+// Rows (one per thread) in wavefront form.  Each thread's window lives in
+// its own column of the LDS tile; consecutive blocks share NQ-1 entries
+// (block ib-1's entries [R, R+NQ-1) are block ib's [0, NQ-1)), which are
+// carried in the tile, so each later block loads R fresh entries.  AB: rows
+// of A (blockIdx.y = 0) and B (1) above the generation window; otherwise all
+// rows of Q (0) and Z (1).
+template <int R, int NQ, bool AB>
+__global__ __launch_bounds__(64) void k_s2_rows_w(
+    double* __restrict__ A, int lda, double* __restrict__ B, int ldb,
+    double* __restrict__ Q, int ldq, double* __restrict__ Z, int ldz, int n,
+    int s, int nblocks, const double* __restrict__ CU,
+    const double* __restrict__ SU, const double* __restrict__ CV,
+    const double* __restrict__ SV) {
+  constexpr int NW = R + NQ - 1;
+  __shared__ double tile[NW][65];
+  __shared__ TapeBlock<R, NQ> tb;
+  const int which = AB ? blockIdx.y : 2 + blockIdx.y;  // 0 A, 1 B, 2 Q, 3 Z
+  const int row_first = blockIdx.x * 64;
+  const int t = threadIdx.x;
+  const int i = row_first + t;  // 0-based row
+  const bool valid = i < n;
+  double* M = which == 0 ? A : which == 1 ? B : which == 2 ? Q : Z;
+  const int ld = which == 0 ? lda : which == 1 ? ldb : which == 2 ? ldq : ldz;
+  const double* Ct = which == 2 ? CU : CV;
+  const double* St = which == 2 ? SU : SV;
+  const double sgn = which == 2 ? -1.0 : 1.0;
+  bool carry = false;
+  for (int ib = nblocks; ib >= 1; --ib) {
+    const int cw0 = s + (ib - 1) * R;  // 0-based first window column
+    const int W = imin(NW, n - cw0);
+    if (W <= 0) {
+      carry = false;
+      continue;
+    }
+    // A, B: rows at or below istart(nq, ib) are the generation's, and
+    // istart only falls with ib
+    if (AB && row_first + 1 >= imax(s + 1, s + 1 + (ib - 1) * R)) break;
+    __syncthreads();  // the previous block's coefficients are consumed
+    stage_tape<R, NQ, 64>(tb, Ct, St, ib, sgn);
+    double* src = M + PHT_IDX(valid ? i : 0, cw0, ld);
+    if (carry) {
+#pragma unroll
+      for (int k = 0; k < NQ - 1; ++k) tile[R + k][t] = tile[k][t];
+    }
+    const int kf = carry ? R : NW;  // fresh entries [0, kf)
+#pragma unroll
+    for (int k0 = 0; k0 < NW; k0 += 16) {
+      double tmp[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        tmp[k] = (valid && k0 + k < W && k0 + k < kf) ? src[(size_t)(k0 + k) * ld] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (k0 + k < NW && k0 + k < kf) tile[k0 + k][t] = tmp[k];
+    }
+    __syncthreads();
+    int icm0 = valid ? 0 : NQ;
+    if (AB && valid) {
+      // sequence icm reaches this row iff i+1 < istart(icm), which rises with icm
+      icm0 = NQ;
+#pragma unroll
+      for (int icm = NQ - 1; icm >= 0; --icm)
+        if (i + 1 < imax(s + 1, s + 1 + (ib - NQ + icm) * R)) icm0 = icm;
+    }
+    if (icm0 < NQ) wavefront_window<R, NQ>(tile, t, tb, icm0);
+    if (valid)
+      for (int k = 0; k < W; ++k) src[(size_t)k * ld] = tile[k][t];
+    carry = true;
+  }
+}
+
+// Columns (one per thread) of A (which = 0) or B (1) right of the generation
+// window, in wavefront form (left rotations in row form: the transposed
+// coefficients).  Global accesses run along the columns through the tile;
+// the tile carries the shared entries between blocks as in k_s2_rows_w.
+template <int R, int NQ>
+__global__ __launch_bounds__(64) void k_s2_cols_w(
+    double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
+    int s, int nblocks, const double* __restrict__ CU,
+    const double* __restrict__ SU) {
+  constexpr int NW = R + NQ - 1;
+  static_assert(NW <= 128, "two 64-row chunks per column");
+  __shared__ double tile[NW][65];
+  __shared__ TapeBlock<R, NQ> tb;
+  const int which = blockIdx.y;
+  const int t = threadIdx.x;
+  const int j0 = blockIdx.x * 64;
+  const int j = j0 + t;
+  double* M = which == 0 ? A : B;
+  const int ld = which == 0 ? lda : ldb;
+  const int cmax = imin(64, n - j0);
+  bool carry = false;
+  for (int ib = nblocks; ib >= 1; --ib) {
+    const int istop = (which == 0) ? ((ib == 1) ? s + NQ - 1 : s + (ib - 2) * R + NQ)
+                                   : s + (ib - 1) * R + R + NQ - 1;  // 1-based
+    if (j0 + 64 <= istop) {  // no column of this workgroup takes part
+      carry = false;
+      continue;
+    }
+    const int rw0 = s + (ib - 1) * R;  // 0-based first row
+    const int W = imin(NW, n - rw0);
+    if (W <= 0) {
+      carry = false;
+      continue;
+    }
+    __syncthreads();  // the previous block's coefficients and tile are consumed
+    stage_tape<R, NQ, 64>(tb, CU, SU, ib, -1.0);
+    if (carry) {
+#pragma unroll
+      for (int k = 0; k < NQ - 1; ++k) tile[R + k][t] = tile[k][t];
+    }
+    const int kf = carry ? R : NW;  // fresh rows [0, kf)
+    // 8 columns x 2 row chunks per group, all 16 loads in flight together
+#pragma unroll
+    for (int c0 = 0; c0 < 64; c0 += 8) {
+      double tmp[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int c = c0 + (q >> 1), k = t + 64 * (q & 1);
+        tmp[q] = (c < cmax && k < W && k < kf) ? M[PHT_IDX(rw0 + k, j0 + c, ld)] : 0.0;
+      }
+      // the carry copy above reads rows [0, NQ-1) of this thread's column
+      // only, so the fresh rows may overwrite other columns' rows now
+      if (c0 == 0) __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int c = c0 + (q >> 1), k = t + 64 * (q & 1);
+        if (k < NW && k < kf) tile[k][c] = tmp[q];
+      }
+    }
+    __syncthreads();
+    const bool act = (j < n) && (j + 1 > istop);
+    if (act) wavefront_window<R, NQ>(tile, t, tb, 0);
+    __syncthreads();
+    for (int c = 0; c < cmax; ++c) {
+      if (j0 + c + 1 <= istop) continue;
+      for (int k = t; k < W; k += 64) M[PHT_IDX(rw0 + k, j0 + c, ld)] = tile[k][c];
+    }
+    carry = true;
+  }
+}
+
+// windows up to 80 entries in the wavefront kernels
+constexpr bool wavefront_fits(int R, int NQ) { return R + NQ - 1 <= 80; }
+
 // (r, nq) pairs with register-window kernels; others use the LDS kernels
 #define PHT_S2_PAIRS(X) \
   X(8, 4) X(8, 8) X(16, 4) X(16, 8) X(16, 16) X(32, 8) X(32, 16) X(32, 32) \
@@ -591,6 +838,17 @@ hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
 #define PHT_ROWS(RR, NN)                                                        \
   if (P.r == RR && P.nq == NN) {                                                \
     const dim3 g((P.n + kRowsThreads - 1) / kRowsThreads, 2);                   \
+    if (wavefront_fits(RR, NN) && s2_wavefront()) {                             \
+      if (part != kRowsQZ)                                                      \
+        hipLaunchKernelGGL((k_s2_rows_w<RR, NN, true>), g, dim3(64), 0, st, A,  \
+                           lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
+                           P.nblocks, CU, SU, CV, SV);                          \
+      if (part != kRowsAB)                                                      \
+        hipLaunchKernelGGL((k_s2_rows_w<RR, NN, false>), g, dim3(64), 0, st, A, \
+                           lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
+                           P.nblocks, CU, SU, CV, SV);                          \
+      return hipGetLastError();                                                 \
+    }                                                                           \
     if (part != kRowsQZ)                                                        \
       hipLaunchKernelGGL((k_s2_rows_v<RR, NN, true>), g, dim3(kRowsThreads), 0, \
                          st, A, lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,     \
@@ -625,9 +883,14 @@ hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
   }
 #define PHT_COLS(RR, NN)                                                     \
   if (P.r == RR && P.nq == NN) {                                             \
-    hipLaunchKernelGGL((k_s2_cols_v<RR, NN>), dim3((P.n + 63) / 64, 2),      \
-                       dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,       \
-                       P.nblocks, CU, SU);                                   \
+    if (wavefront_fits(RR, NN) && s2_wavefront())                            \
+      hipLaunchKernelGGL((k_s2_cols_w<RR, NN>), dim3((P.n + 63) / 64, 2),    \
+                         dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,     \
+                         P.nblocks, CU, SU);                                 \
+    else                                                                     \
+      hipLaunchKernelGGL((k_s2_cols_v<RR, NN>), dim3((P.n + 63) / 64, 2),    \
+                         dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,     \
+                         P.nblocks, CU, SU);                                 \
     return hipGetLastError();                                                \
   }
   PHT_S2_PAIRS(PHT_COLS)
