@@ -80,41 +80,102 @@ __device__ __forceinline__ double grp_norm(const double* X, Sel sel) {
 }
 
 // G(x,y) = v_x . v_y (x < y < k) for reflectors stored as zero-padded rows of
-// Vs (kLd apart, 128 entries), then T (kfull x kfull, ld ldt, zeros below the
+// Vs (ldvs apart, 128 entries), then T (kfull x kfull, ld ldt, zeros below the
 // diagonal and beyond k) of H = H(0) ... H(k-1) = I - V T V^T.  Called by all
 // kT threads after a barrier that completed Vs and tau.
+//
+// T is built from 16 x 16 blocks: the four diagonal blocks by the DLARFT
+// forward recurrence T(0:j-1,j) = -tau_j T(0:j-1,0:j-1) G(0:j-1,j) (one wave
+// each, a lane owning a row in registers), then the compact-WY product rule
+// for H = H_A H_B, T_AB = -T_AA G_AB T_BB, on FP64 MFMA: first within the two
+// 32-column halves, then between them.  Entries beyond k are zero (their
+// Gram rows and tau are masked).
 __device__ inline void gram_and_t(const double* Vs, int k, const double* tau, double* G,
-                           double* Tl, double* Tout, int ldt, int kfull) {
+                           double* Tl, double* Tout, int ldt, int kfull, int ldvs = kLd) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int l16 = lane & 15, q = lane >> 4;
   {
     // one 16x16 tile of G per wave: rows 16*I.., columns 16*J.. (I <= J)
     const int I = wave >> 2, J = wave & 3;
-    if (I <= J && 16 * I < k) {
-      const double* va = Vs + (16 * I + l16) * kLd;
-      const double* vb = Vs + (16 * J + l16) * kLd;
+    if (I <= J) {
+      const bool oka = 16 * I + l16 < k, okb = 16 * J + l16 < k;
+      const double* va = Vs + (16 * I + (oka ? l16 : 0)) * ldvs;
+      const double* vb = Vs + (16 * J + (okb ? l16 : 0)) * ldvs;
       dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 8
-      for (int k0 = 0; k0 < 128; k0 += 4) acc = mfma16(va[k0 + q], vb[k0 + q], acc);
+      for (int k0 = 0; k0 < 128; k0 += 4)
+        acc = mfma16(oka ? va[k0 + q] : 0.0, okb ? vb[k0 + q] : 0.0, acc);
 #pragma unroll
       for (int i = 0; i < 4; ++i) G[(16 * I + 4 * i + q) * 65 + 16 * J + l16] = acc[i];
     }
   }
   __syncthreads();
-  if (threadIdx.x < 64) {
-    // lane a computes row a of T; it reads only its own row and G
-    const int a = threadIdx.x;
-    if (a < k) Tl[a * 65 + a] = tau[a];
-    for (int j = a + 1; j < k; ++j) {
+  if (wave < 4 && lane < 16) {
+    // diagonal block w: lane a holds row a of T(base.., base..)
+    const int base = 16 * wave, a = lane;
+    double trow[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) trow[l] = (l == a && base + l < k) ? tau[base + l] : 0.0;
+#pragma unroll
+    for (int j = 1; j < 16; ++j) {
+      double g[16];
+#pragma unroll
+      for (int l = 0; l < j; ++l) g[l] = G[(base + l) * 65 + base + j];
       double s0 = 0.0, s1 = 0.0;
-      int l = a;
-      for (; l + 1 < j; l += 2) {
-        s0 += Tl[a * 65 + l] * G[l * 65 + j];
-        s1 += Tl[a * 65 + l + 1] * G[(l + 1) * 65 + j];
+#pragma unroll
+      for (int l = 0; l < j; l += 2) {
+        s0 += trow[l] * g[l];
+        if (l + 1 < j) s1 += trow[l + 1] * g[l + 1];
       }
-      if (l < j) s0 += Tl[a * 65 + l] * G[l * 65 + j];
-      Tl[a * 65 + j] = -tau[j] * (s0 + s1);
+      if (j > a && base + j < k) trow[j] = -tau[base + j] * (s0 + s1);
     }
+#pragma unroll
+    for (int l = 0; l < 16; ++l) Tl[(base + a) * 65 + base + l] = trow[l];
+  }
+  __syncthreads();
+  if (wave < 2) {
+    // T(b0, b1) = -T(b0, b0) [G(b0, b1) T(b1, b1)] for the blocks of a half
+    const int b0 = 32 * wave, b1 = b0 + 16;
+    dbl4 w = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      w = mfma16(G[(b0 + l16) * 65 + b1 + 4 * i + q], Tl[(b1 + 4 * i + q) * 65 + b1 + l16], w);
+    dbl4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t = mfma16(Tl[(b0 + l16) * 65 + b0 + 4 * i + q], w[i], t);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Tl[(b0 + 4 * i + q) * 65 + b1 + l16] = -t[i];
+  }
+  __syncthreads();
+  // T(0:32, 32:64) = -T(0:32, 0:32) [G(0:32, 32:64) T(32:64, 32:64)]; the
+  // bracket's block (I, J) goes to G's unused block (J, I)
+  const int I = wave >> 1, J = 2 + (wave & 1);
+  if (wave < 4) {
+    dbl4 w = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int K = 2; K < 4; ++K) {
+      if (K > J) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        w = mfma16(G[(16 * I + l16) * 65 + 16 * K + 4 * i + q],
+                   Tl[(16 * K + 4 * i + q) * 65 + 16 * J + l16], w);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) G[(16 * J + 4 * i + q) * 65 + 16 * I + l16] = w[i];
+  }
+  __syncthreads();
+  if (wave < 4) {
+    dbl4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int K = 0; K < 2; ++K) {
+      if (K < I) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        t = mfma16(Tl[(16 * I + l16) * 65 + 16 * K + 4 * i + q],
+                   G[(16 * J + 4 * i + q) * 65 + 16 * K + l16], t);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Tl[(16 * I + 4 * i + q) * 65 + 16 * J + l16] = -t[i];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < kfull * kfull; e += kT) {
