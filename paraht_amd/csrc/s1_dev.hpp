@@ -285,6 +285,7 @@ __device__ inline void rq_gen_body(
   double* Ys = Vr;                  // [64][kLd]: LQ reflectors, after the Y phase
   double* G = Vr + 64 * kLd;        // [64][65]
   double* Tl = G + 64 * 65;         // [64][65] (ends inside taus, dead by then)
+  double* Tb = tau2 + 64;           // [8][16][17]: T of each block of 16 RQ reflectors
   const int tid = threadIdx.x;
 
   // ---- DGERQ2 on the len x len block: reflector i from row i, zero right of
@@ -358,32 +359,112 @@ __device__ inline void rq_gen_body(
     return;
   }
 
-  // ---- Y = E^T H(0) H(1) ... H(len-1): the first nb rows of the RQ's Q
+  // ---- Y = E^T H(0) H(1) ... H(len-1): the first nb rows of the RQ's Q, in
+  // blocks of 16 reflectors.  Block m = H(16m) ... H(16m+15) = I - V^T T V
+  // (forward T); Y^T <- (I - V^T T^T V) Y^T for m = 0, 1, ...  First one wave
+  // per block builds T from the block's Gram matrix (on MFMA, into the block's
+  // LDS slot, which then receives T); then one wave per 16 columns of Y^T,
+  // held as 8 MFMA accumulator tiles (element i of tile ct: row 16 ct + 4 i
+  // + lane/16, column lane%16), applies the blocks in order.
+  {
+    const int wave = tid >> 6, lane = tid & 63, l16 = lane & 15, qq = lane >> 4;
+    const int nblk = (len + 15) >> 4;
+    if (wave < nblk) {
+      const int b = 16 * wave;
+      double* slot = Tb + wave * 272;
+      const bool ok = b + l16 < len;
+      const double* va = Vr + (ok ? b + l16 : 0) * kLd;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+      for (int k0 = 0; k0 < 128; k0 += 4) {
+        const double x = ok ? va[k0 + qq] : 0.0;
+        acc = mfma16(x, x, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) slot[(4 * i + qq) * 17 + l16] = acc[i];
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 16) {
+        const int a0 = lane;
+        double trow[16];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) trow[l] = (l == a0 && b + l < len) ? taus[b + l] : 0.0;
+#pragma unroll
+        for (int j = 1; j < 16; ++j) {
+          double g[16];
+#pragma unroll
+          for (int l = 0; l < j; ++l) g[l] = slot[l * 17 + j];
+          double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+          for (int l = 0; l < j; l += 2) {
+            s0 += trow[l] * g[l];
+            if (l + 1 < j) s1 += trow[l + 1] * g[l + 1];
+          }
+          if (j > a0 && b + j < len) trow[j] = -taus[b + j] * (s0 + s1);
+        }
+        // every lane's Gram reads precede these writes in the wave's order
+#pragma unroll
+        for (int l = 0; l < 16; ++l) slot[a0 * 17 + l] = trow[l];
+      }
+    }
+    __syncthreads();
+    dbl4 Yt[8];
+    if (wave < ((nb + 15) >> 4)) {
+      const int col = 16 * wave + l16;
+#pragma unroll
+      for (int ct = 0; ct < 8; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          Yt[ct][i] = (16 * ct + 4 * i + qq == col && col < nb) ? 1.0 : 0.0;
+      for (int m = 0; m < nblk; ++m) {
+        const int b = 16 * m;  // the block's entries lie in columns [0, b+16)
+        const double* slot = Tb + m * 272;
+        const bool vok = b + l16 < len;
+        const double* va = Vr + (vok ? b + l16 : 0) * kLd;
+        dbl4 wt = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct) {
+          if (ct > m) continue;
+          double av[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) av[i] = vok ? va[16 * ct + 4 * i + qq] : 0.0;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) wt = mfma16(av[i], Yt[ct][i], wt);
+        }
+        double tv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tv[i] = slot[(4 * i + qq) * 17 + l16];
+        dbl4 w2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w2 = mfma16(tv[i], wt[i], w2);
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct) {
+          if (ct > m) continue;
+          double av[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int kk = 4 * i + qq;
+            av[i] = (b + kk < len) ? -Vr[(b + kk) * kLd + 16 * ct + l16] : 0.0;
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) Yt[ct] = mfma16(av[i], w2[i], Yt[ct]);
+        }
+      }
+    }
+    __syncthreads();  // Ys overwrites Vr from here on
+    if (wave < ((nb + 15) >> 4)) {
+      double* yrow = Ys + (16 * wave + l16) * kLd;
+#pragma unroll
+      for (int ct = 0; ct < 8; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) yrow[16 * ct + 4 * i + qq] = Yt[ct][i];
+    }
+    __syncthreads();
+  }
   constexpr int NJ = 8;
   const int a = tid >> 4, q = tid & 15;
   double Y[NJ];
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) Y[j] = (a < nb && 16 * j + q == a) ? 1.0 : 0.0;
-  if (a < nb) {
-    for (int i = 1; i < len; ++i) {  // H(0) is the identity (length-1 reflector)
-      const double t = taus[i];
-      if (t == 0.0) continue;
-      const double* vi = Vr + i * kLd;
-      double w = 0.0;
-#pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        if (16 * j > i) continue;
-        w += Y[j] * vi[16 * j + q];
-      }
-      w = grp_sum<16>(w) * t;
-#pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        if (16 * j > i) continue;
-        Y[j] -= w * vi[16 * j + q];
-      }
-    }
-  }
-  __syncthreads();  // Ys overwrites Vr from here on
+  for (int j = 0; j < NJ; ++j) Y[j] = (a < nb) ? Ys[a * kLd + 16 * j + q] : 0.0;
 
   // ---- DGELQ2 on Y (nb x len): reflector i from row i, zero left of column
   // i, applied to the rows below it.  Its LDS row is the explicit V row.
@@ -435,7 +516,7 @@ __device__ inline void rq_gen_body(
 
 
 constexpr size_t kPanelLds = (size_t)(64 * kLd + 2 * 64 * 65 + 64) * sizeof(double);
-constexpr size_t kRqLds = (size_t)(128 * kLd + 128 + 64) * sizeof(double);
+constexpr size_t kRqLds = (size_t)(128 * kLd + 128 + 64 + 8 * 272) * sizeof(double);
 static_assert(kPanelLds <= 160 * 1024 && kRqLds <= 160 * 1024, "LDS budget");
 static_assert(64 * kLd + 2 * 64 * 65 <= 128 * kLd + 128, "Gram/T alias in the RQ kernel");
 
