@@ -676,7 +676,12 @@ __global__ __launch_bounds__(64) void k_s2_rows_w(
       for (int icm = NQ - 1; icm >= 0; --icm)
         if (i + 1 < imax(s + 1, s + 1 + (ib - NQ + icm) * R)) icm0 = icm;
     }
-    if (icm0 < NQ) wavefront_window<R, NQ>(tile, t, tb, icm0);
+    // every sequence reaches every row of the wave (the common case): the
+    // variant without per-rotation masks
+    if (__all(icm0 == 0))
+      wavefront_window<R, NQ>(tile, t, tb, 0);
+    else if (icm0 < NQ)
+      wavefront_window<R, NQ>(tile, t, tb, icm0);
     if (valid)
       for (int k = 0; k < W; ++k) src[(size_t)k * ld] = tile[k][t];
     carry = true;
