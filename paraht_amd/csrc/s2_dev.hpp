@@ -73,19 +73,32 @@ __device__ __forceinline__ void right_chain(double (*Bb)[RMAX + 1], int nb, int 
   const bool row = lane < nb;
   double cur = row ? Bb[lane][nb - 1] : 0.0;
   double myc = 1.0, mys = 0.0;  // the rotation pivoted on this lane
-#pragma unroll 4
-  for (int ig = nb - 1; ig >= 1; --ig) {
-    const double y = row ? Bb[lane][ig - 1] : 0.0;
-    const double f = readlane_d(cur, ig);
-    const double g = readlane_d(y, ig);
-    double c, sn, rr;
-    rot_rsq(f, g, c, sn, rr);
-    const double nx = c * cur + sn * y, ny = c * y - sn * cur;
-    const bool piv = lane == ig, above = lane < ig;
-    if (row) Bb[lane][ig] = piv ? rr : (above ? nx : cur);
-    cur = piv ? 0.0 : (above ? ny : y);
-    myc = piv ? c : myc;
-    mys = piv ? sn : mys;
+  // chunks of 8 rotations: column ig-1 is only written by rotation ig-1,
+  // after rotation ig read it, so a chunk's original columns are read up
+  // front and the chain never waits on LDS
+  for (int ig0 = nb - 1; ig0 >= 1; ig0 -= 8) {
+    double yv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int ig = ig0 - u;
+      yv[u] = (row && ig >= 1) ? Bb[lane][ig >= 1 ? ig - 1 : 0] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int ig = ig0 - u;
+      if (ig < 1) break;
+      const double y = yv[u];
+      const double f = readlane_d(cur, ig);
+      const double g = readlane_d(y, ig);
+      double c, sn, rr;
+      rot_rsq(f, g, c, sn, rr);
+      const double nx = c * cur + sn * y, ny = c * y - sn * cur;
+      const bool piv = lane == ig, above = lane < ig;
+      if (row) Bb[lane][ig] = piv ? rr : (above ? nx : cur);
+      cur = piv ? 0.0 : (above ? ny : y);
+      myc = piv ? c : myc;
+      mys = piv ? sn : mys;
+    }
   }
   if (row) Bb[lane][0] = cur;
   if (lane >= 1 && lane < nb) {

@@ -29,7 +29,7 @@
 namespace pht {
 
 namespace {
-constexpr int kGenThreads = 512;
+constexpr int kGenThreads = 256;
 constexpr int kTile = 128;
 // far updates in wavefront form (PARAHT_S2_WAVEFRONT=0: the sequential form)
 bool s2_wavefront() {
