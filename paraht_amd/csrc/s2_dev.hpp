@@ -154,18 +154,33 @@ __device__ inline void chase_step(double* __restrict__ A, int lda, double* __res
   const int L = imin(ldr - 1, n - base);
   double* M = Mw + (size_t)(ib - 1) * msz;
 
-  // ---- M of this block into LDS (identity for the first column)
-  for (int e0 = 0; e0 < ldr * ldr; e0 += 16 * NT) {
-    double tmp[16];
+  // ---- M of this block into LDS (identity for the first column): one wave
+  //      per set of columns, lanes along the rows (ldr <= kMaxLdr = 160 rows:
+  //      three 64-row chunks), 8 columns' loads in flight together
+  {
+    constexpr int NWV = NT / 64;
+    for (int cm0 = wave; cm0 < ldr; cm0 += 8 * NWV) {
+      double tmp[8][3];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int e = e0 + tid + q * NT;
-      tmp[q] = (ic > 1 && e < ldr * ldr) ? M[e] : (e % ldr == e / ldr ? 1.0 : 0.0);
-    }
+      for (int u = 0; u < 8; ++u) {
+        const int cm = cm0 + u * NWV;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int e = e0 + tid + q * NT;
-      if (e < ldr * ldr) Ml[(e / ldr) * ldl + e % ldr] = tmp[q];
+        for (int h = 0; h < 3; ++h) {
+          const int rw = lane + 64 * h;
+          tmp[u][h] = (cm < ldr && rw < ldr)
+                          ? ((ic > 1) ? M[(size_t)cm * ldr + rw] : (rw == cm ? 1.0 : 0.0))
+                          : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int cm = cm0 + u * NWV;
+#pragma unroll
+        for (int h = 0; h < 3; ++h) {
+          const int rw = lane + 64 * h;
+          if (cm < ldr && rw < ldr) Ml[cm * ldl + rw] = tmp[u][h];
+        }
+      }
     }
   }
   __syncthreads();
