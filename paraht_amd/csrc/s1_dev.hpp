@@ -9,11 +9,17 @@
 //    j of a thread in group position q being index G*j + q, so the lanes that
 //    read one LDS vector touch consecutive words.  Building a reflector (the
 //    serial part of every step) costs a thread 8-16 entries, not a row.
-//  * Every reflector is stored zero-padded to 128 entries in its own LDS row,
-//    which is both the broadcast buffer of its update and the explicit V of
-//    the block (one barrier per reflector, no masks in the update loops).
-//  * The work of a reflector is bounded by skipping whole chunks on a
-//    workgroup-uniform test instead of per-element predicates.
+//  * The reflector loops run over the chunk index j of the pivot (static, the
+//    outer loop is unrolled) and the group position inside the chunk
+//    (runtime), so every register index and every chunk bound is a
+//    compile-time constant: no select chains to find the pivot entry, no
+//    runtime tests to skip the chunks a reflector does not reach.
+//  * A reflector goes to one of two LDS broadcast rows (one barrier per
+//    reflector); the lines keep their final entries in registers, and the
+//    explicit V rows (zero-padded to 128) are written once after the loop.
+//  * The reflector comes from the group's sum of squares with v_rsq_f64 and
+//    v_rcp_f64 refined by Newton steps (refl_from_ss); DLARFG's scaled path
+//    takes over when the sum of squares leaves the safe range.
 //  * G = V^T V runs on FP64 MFMA (one 16x16 tile per wave); T comes from the
 //    DLARFT forward recurrence T(0:j-1,j) = -tau_j T(0:j-1,0:j-1) G(0:j-1,j),
 //    each lane of one wave owning a row of T (no barriers).
@@ -77,6 +83,44 @@ __device__ __forceinline__ double grp_norm(const double* X, Sel sel) {
     s2 += t * t;
   }
   return mx * sqrt(grp_sum<G>(s2));
+}
+
+// DLARFG from alpha and ss = ||x||^2 (already summed over the group):
+// beta = -sign(alpha) sqrt(alpha^2 + ss), tau = (beta - alpha) / beta =
+// 1 + |alpha| / |beta|, scal = 1 / (alpha - beta) = sign(alpha) / (|alpha| + |beta|).
+// 1/|beta| is v_rsq_f64 of alpha^2 + ss refined by two Newton steps, and
+// 1/(|alpha| + |beta|) v_rcp_f64 refined by two more (about 1 ulp each).
+// Outside the safe range (ss tiny or huge, |alpha| huge) the scaled norm and
+// make_refl's rescaling loop take over, as DLARFG does.
+template <int G, int NJ, typename Sel>
+__device__ __forceinline__ Refl refl_from_ss(double al, double ss, const double* X, Sel sel) {
+  Refl R;
+  if (ss == 0.0) {
+    R.tau = 0.0;
+    R.scal = 1.0;
+    R.beta = al;
+    return R;
+  }
+  if (ss > 1e-290 && ss < 1e290 && fabs(al) < 1e140) {
+    const double n2 = fma(al, al, ss);
+    double y = __builtin_amdgcn_rsq(n2);
+    double e = fma(-n2 * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-n2 * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    const double nrm = n2 * y, aa = fabs(al);
+    R.tau = fma(aa, y, 1.0);
+    const double d = aa + nrm;
+    double z = __builtin_amdgcn_rcp(d);
+    double f = fma(-d, z, 1.0);
+    z = fma(z, f, z);
+    f = fma(-d, z, 1.0);
+    z = fma(z, f, z);
+    R.scal = copysign(z, al);
+    R.beta = -copysign(nrm, al);
+    return R;
+  }
+  return make_refl(al, grp_norm<G, NJ>(X, sel));
 }
 
 // G(x,y) = v_x . v_y (x < y < k) for reflectors stored as zero-padded rows of
@@ -195,6 +239,7 @@ __device__ inline void panel_qr_body(
   double* G = Vc + 64 * kLd;     // [64][65]
   double* Tl = G + 64 * 65;      // [64][65]
   double* tau = Tl + 64 * 65;    // [64]
+  double* bc = tau + 64;         // [2][128]: broadcast rows of the reflectors
   const int tid = threadIdx.x;
   const int col = tid >> 4, q = tid & 15;
   const int P = p * nb;
@@ -209,52 +254,71 @@ __device__ inline void panel_qr_body(
       const int r = 16 * j + q;
       X[j] = (col < nb && r < len) ? A[PHT_IDX(i0 + r, jb + col, lda)] : 0.0;
     }
-    // DGEQR2: reflector i from column i (16 lanes), applied to the columns
-    // right of it; it is zero above row i.
-    for (int i = 0; i < k; ++i) {
-      double* vi = Vc + i * kLd;
-      if (col == i) {
-        double al = 0.0;
+    // DGEQR2: reflector i = 16 jj + qp from column i (16 lanes), applied to
+    // the columns right of it; it is zero above row i, so only chunks j >= jj
+    // take part.
+    double mysc = 1.0, mybeta = 0.0;
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) al += (16 * j + q == i) ? X[j] : 0.0;
-        al = grp_sum<16>(al);
-        const double xnorm = grp_norm<16, NJ>(X, [&](int j) { return 16 * j + q > i; });
-        const Refl R = make_refl(al, xnorm);
-        const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+    for (int jj = 0; jj < NJ; ++jj) {
+      for (int qp = 0; qp < 16; ++qp) {
+        const int i = 16 * jj + qp;
+        if (i >= k) break;
+        double* vb = bc + (i & 1) * 128;
+        if (col == i) {
+          const double al = grp_sum<16>(q == qp ? X[jj] : 0.0);
+          double s0 = (q > qp) ? X[jj] * X[jj] : 0.0, s1 = 0.0;
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-          const int r = 16 * j + q;
-          const double v = (r > i) ? X[j] * sc : (r == i ? 1.0 : 0.0);
-          vi[r] = v;
-          X[j] = (r > i) ? v : (r == i ? R.beta : X[j]);
+          for (int j = jj + 1; j < NJ; j += 2) {
+            s0 += X[j] * X[j];
+            if (j + 1 < NJ) s1 += X[j + 1] * X[j + 1];
+          }
+          const double ss = grp_sum<16>(s0 + s1);
+          const Refl R = refl_from_ss<16, NJ>(al, ss, X, [&](int j) { return 16 * j + q > i; });
+          const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+          mysc = sc;
+          mybeta = R.beta;
+          vb[16 * jj + q] = (q > qp) ? X[jj] * sc : (q == qp ? 1.0 : 0.0);
+#pragma unroll
+          for (int j = jj + 1; j < NJ; ++j) vb[16 * j + q] = X[j] * sc;
+          if (q == 0) tau[i] = R.tau;
         }
-        if (q == 0) tau[i] = R.tau;
-      }
-      __syncthreads();
-      const double t = tau[i];
-      if (col > i && col < nb && t != 0.0) {
-        double w = 0.0;
+        __syncthreads();
+        const double t = tau[i];
+        if (col > i && col < nb && t != 0.0) {
+          double v[NJ];
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-          if (16 * j + 15 < i) continue;  // entries above the reflector
-          w += X[j] * vi[16 * j + q];
-        }
-        w = grp_sum<16>(w) * t;
+          for (int j = jj; j < NJ; ++j) v[j] = vb[16 * j + q];
+          double w0 = 0.0, w1 = 0.0;
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-          if (16 * j + 15 < i) continue;
-          X[j] -= w * vi[16 * j + q];
+          for (int j = jj; j < NJ; j += 2) {
+            w0 += X[j] * v[j];
+            if (j + 1 < NJ) w1 += X[j + 1] * v[j + 1];
+          }
+          const double w = grp_sum<16>(w0 + w1) * t;
+#pragma unroll
+          for (int j = jj; j < NJ; ++j) X[j] -= w * v[j];
         }
       }
     }
-    // R (upper trapezoid) and zeros back into A
+    // R (upper trapezoid) and zeros back into A; the explicit reflector rows
+    // (zero above the diagonal, 1 on it) into Vc
     if (col < nb) {
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
         const int r = 16 * j + q;
-        if (r < len) A[PHT_IDX(i0 + r, jb + col, lda)] = (r <= col) ? X[j] : 0.0;
+        if (r < len)
+          A[PHT_IDX(i0 + r, jb + col, lda)] =
+              (r < col) ? X[j] : (r == col ? (col < k ? mybeta : X[j]) : 0.0);
       }
     }
+    if (col < k) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int r = 16 * j + q;
+        Vc[col * kLd + r] = (r > col) ? X[j] * mysc : (r == col ? 1.0 : 0.0);
+      }
+    }
+    __syncthreads();
     // explicit V to the tape (coalesced along the rows)
     double* Vb = Vt + (long long)bl * vstride;
     for (int e = tid; e < len * nb; e += kT) {
@@ -288,52 +352,70 @@ __device__ inline void rq_gen_body(
   double* Tb = tau2 + 64;           // [8][16][17]: T of each block of 16 RQ reflectors
   const int tid = threadIdx.x;
 
-  // ---- DGERQ2 on the len x len block: reflector i from row i, zero right of
-  // column i, applied to the rows above it.
+  // ---- DGERQ2 on the len x len block: reflector i = 8 jj + hp from row i,
+  // zero right of column i, applied to the rows above it; only chunks
+  // j <= jj take part.  The broadcast rows live in Tb (free until the Y phase).
   {
     constexpr int NJ = 16;
     const int row = tid >> 3, h = tid & 7;
+    double* bc = Tb;
     double X[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       const int c = 8 * j + h;
       X[j] = (row < len && c < len) ? B[PHT_IDX(i0 + row, i0 + c, ldb)] : 0.0;
     }
-    for (int i = len - 1; i >= 0; --i) {
-      double* vi = Vr + i * kLd;
-      if (row == i) {
-        double al = 0.0;
+    double mysc = 1.0;
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) al += (8 * j + h == i) ? X[j] : 0.0;
-        al = grp_sum<8>(al);
-        const double xnorm = grp_norm<8, NJ>(X, [&](int j) { return 8 * j + h < i; });
-        const Refl R = make_refl(al, xnorm);
-        const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+    for (int jj = NJ - 1; jj >= 0; --jj) {
+      for (int hp = 7; hp >= 0; --hp) {
+        const int i = 8 * jj + hp;
+        if (i >= len) continue;
+        double* vb = bc + (i & 1) * 128;
+        if (row == i) {
+          const double al = grp_sum<8>(h == hp ? X[jj] : 0.0);
+          double s0 = (h < hp) ? X[jj] * X[jj] : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-          const int c = 8 * j + h;
-          vi[c] = (c < i) ? X[j] * sc : (c == i ? 1.0 : 0.0);
+          for (int j = 0; j < jj; j += 4) {
+            s0 += X[j] * X[j];
+            if (j + 1 < jj) s1 += X[j + 1] * X[j + 1];
+            if (j + 2 < jj) s2 += X[j + 2] * X[j + 2];
+            if (j + 3 < jj) s3 += X[j + 3] * X[j + 3];
+          }
+          const double ss = grp_sum<8>((s0 + s1) + (s2 + s3));
+          const Refl R = refl_from_ss<8, NJ>(al, ss, X, [&](int j) { return 8 * j + h < i; });
+          const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+          mysc = sc;
+#pragma unroll
+          for (int j = 0; j < jj; ++j) vb[8 * j + h] = X[j] * sc;
+          vb[8 * jj + h] = (h < hp) ? X[jj] * sc : (h == hp ? 1.0 : 0.0);
+          if (h == 0) taus[i] = R.tau;
         }
-        if (h == 0) taus[i] = R.tau;
+        __syncthreads();
+        const double t = taus[i];
+        if (row < i && t != 0.0) {
+          double v[NJ];
+#pragma unroll
+          for (int j = 0; j <= jj; ++j) v[j] = vb[8 * j + h];
+          double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+#pragma unroll
+          for (int j = 0; j <= jj; j += 4) {
+            w0 += X[j] * v[j];
+            if (j + 1 <= jj) w1 += X[j + 1] * v[j + 1];
+            if (j + 2 <= jj) w2 += X[j + 2] * v[j + 2];
+            if (j + 3 <= jj) w3 += X[j + 3] * v[j + 3];
+          }
+          const double w = grp_sum<8>((w0 + w1) + (w2 + w3)) * t;
+#pragma unroll
+          for (int j = 0; j <= jj; ++j) X[j] -= w * v[j];
+        }
       }
-      __syncthreads();
-      const double t = taus[i];
-      if (row < i && t != 0.0) {
-        double w0 = 0.0, w1 = 0.0;
+    }
+    // reflector i in row i of Vr: zero right of column i, 1 on it
 #pragma unroll
-        for (int j = 0; j < NJ; j += 2) {
-          if (8 * j > i) continue;  // entries right of the reflector
-          w0 += X[j] * vi[8 * j + h];
-          w1 += X[j + 1] * vi[8 * j + 8 + h];
-        }
-        const double w = grp_sum<8>(w0 + w1) * t;
-#pragma unroll
-        for (int j = 0; j < NJ; j += 2) {
-          if (8 * j > i) continue;
-          X[j] -= w * vi[8 * j + h];
-          X[j + 1] -= w * vi[8 * j + 8 + h];
-        }
-      }
+    for (int j = 0; j < NJ; ++j) {
+      const int c = 8 * j + h;
+      Vr[row * kLd + c] = (row >= len) ? 0.0 : (c < row) ? X[j] * mysc : (c == row ? 1.0 : 0.0);
     }
   }
   __syncthreads();
@@ -466,46 +548,63 @@ __device__ inline void rq_gen_body(
 #pragma unroll
   for (int j = 0; j < NJ; ++j) Y[j] = (a < nb) ? Ys[a * kLd + 16 * j + q] : 0.0;
 
-  // ---- DGELQ2 on Y (nb x len): reflector i from row i, zero left of column
-  // i, applied to the rows below it.  Its LDS row is the explicit V row.
-  for (int i = 0; i < nb; ++i) {
-    double* vi = Ys + i * kLd;
-    if (a == i) {
-      double al = 0.0, ss = 0.0;
+  // ---- DGELQ2 on Y (nb x len): reflector i = 16 jj + qp from row i, zero
+  // left of column i, applied to the rows below it; only chunks j >= jj take
+  // part.  Broadcast rows in Tb (free again after the Y phase).
+  double mysc = 1.0;
+  {
+    double* bc = Tb;
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const int c = 16 * j + q;
-        al += (c == i) ? Y[j] : 0.0;
-        ss += (c > i) ? Y[j] * Y[j] : 0.0;  // rows of an orthogonal factor: |entries| <= 1
-      }
-      al = grp_sum<16>(al);
-      const double xnorm = sqrt(grp_sum<16>(ss));
-      const Refl R = make_refl(al, xnorm);
-      const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+    for (int jj = 0; jj < NJ; ++jj) {
+      for (int qp = 0; qp < 16; ++qp) {
+        const int i = 16 * jj + qp;
+        if (i >= nb) break;
+        double* vb = bc + (i & 1) * 128;
+        if (a == i) {
+          const double al = grp_sum<16>(q == qp ? Y[jj] : 0.0);
+          double s0 = (q > qp) ? Y[jj] * Y[jj] : 0.0, s1 = 0.0;
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const int c = 16 * j + q;
-        vi[c] = (c > i) ? Y[j] * sc : (c == i ? 1.0 : 0.0);
-      }
-      if (q == 0) tau2[i] = R.tau;
-    }
-    __syncthreads();
-    const double t = tau2[i];
-    if (a > i && a < nb && t != 0.0) {
-      double w = 0.0;
+          for (int j = jj + 1; j < NJ; j += 2) {
+            s0 += Y[j] * Y[j];
+            if (j + 1 < NJ) s1 += Y[j + 1] * Y[j + 1];
+          }
+          const double ss = grp_sum<16>(s0 + s1);
+          const Refl R = refl_from_ss<16, NJ>(al, ss, Y, [&](int j) { return 16 * j + q > i; });
+          const double sc = (R.tau != 0.0) ? R.scal : 1.0;
+          mysc = sc;
+          vb[16 * jj + q] = (q > qp) ? Y[jj] * sc : (q == qp ? 1.0 : 0.0);
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        if (16 * j + 15 < i) continue;  // entries left of the reflector
-        w += Y[j] * vi[16 * j + q];
-      }
-      w = grp_sum<16>(w) * t;
+          for (int j = jj + 1; j < NJ; ++j) vb[16 * j + q] = Y[j] * sc;
+          if (q == 0) tau2[i] = R.tau;
+        }
+        __syncthreads();
+        const double t = tau2[i];
+        if (a > i && a < nb && t != 0.0) {
+          double v[NJ];
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        if (16 * j + 15 < i) continue;
-        Y[j] -= w * vi[16 * j + q];
+          for (int j = jj; j < NJ; ++j) v[j] = vb[16 * j + q];
+          double w0 = 0.0, w1 = 0.0;
+#pragma unroll
+          for (int j = jj; j < NJ; j += 2) {
+            w0 += Y[j] * v[j];
+            if (j + 1 < NJ) w1 += Y[j + 1] * v[j + 1];
+          }
+          const double w = grp_sum<16>(w0 + w1) * t;
+#pragma unroll
+          for (int j = jj; j < NJ; ++j) Y[j] -= w * v[j];
+        }
       }
     }
   }
+  // reflector i in row i of Ys: zero left of column i, 1 on it
+  if (a < nb) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int c = 16 * j + q;
+      Ys[a * kLd + c] = (c > a) ? Y[j] * mysc : (c == a ? 1.0 : 0.0);
+    }
+  }
+  __syncthreads();
   // explicit V (len x nb), coalesced along the rows
   for (int e = tid; e < len * nb; e += kT) {
     const int c = e % len, r = e / len;
@@ -515,9 +614,11 @@ __device__ inline void rq_gen_body(
 }
 
 
-constexpr size_t kPanelLds = (size_t)(64 * kLd + 2 * 64 * 65 + 64) * sizeof(double);
+constexpr size_t kPanelLds = (size_t)(64 * kLd + 2 * 64 * 65 + 64 + 256) * sizeof(double);
 constexpr size_t kRqLds = (size_t)(128 * kLd + 128 + 64 + 8 * 272) * sizeof(double);
 static_assert(kPanelLds <= 160 * 1024 && kRqLds <= 160 * 1024, "LDS budget");
+static_assert(kPanelLds <= kRqLds, "the batched kernel sizes its LDS for both bodies by kRqLds");
+static_assert(8 * 272 >= 256, "two broadcast rows fit in the Y phase's T slots");
 static_assert(64 * kLd + 2 * 64 * 65 <= 128 * kLd + 128, "Gram/T alias in the RQ kernel");
 
 }  // namespace s1
