@@ -83,11 +83,21 @@ __global__ void k_s2_prep(double* CU, double* SU, double* CV, double* SV,
   if (gid == 0) *errflag = 0;
 }
 
-// Thread 0 spins (bounded) until *p >= target; the workgroup then acquires.
+// Cross-workgroup hand-off.  An agent-scope acquire or release on gfx950
+// also invalidates (buffer_inv sc1) or writes back (buffer_wbl2 sc1) the
+// L2 of the XCD it runs on, which every other kernel on that XCD pays for.
+// So the spin reads the counter with relaxed agent-scope loads (coherent,
+// no invalidation per poll), thread 0 alone issues one acquire fence once
+// every wait is over (acquire_waits), and a step is published by one
+// release store from thread 0 after every wave's stores have completed
+// (publish).  The workgroup barrier between thread 0 and the other waves
+// carries the ordering to them.
+
+// Thread 0 spins (bounded) until *p >= target.
 __device__ __forceinline__ void wait_at_least(int* p, int target, int* errflag) {
   if (threadIdx.x == 0) {
     int spins = 0;
-    while (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(2);
       if (++spins > kSpinLimit) {
         __hip_atomic_store(errflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -95,6 +105,21 @@ __device__ __forceinline__ void wait_at_least(int* p, int target, int* errflag) 
       }
     }
   }
+}
+
+// After thread 0's waits: one agent-scope acquire, then the workgroup barrier.
+__device__ __forceinline__ void acquire_waits() {
+  if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __syncthreads();
+}
+
+// Every wave waits for its own global stores to complete (reach L2), the
+// barrier orders that before thread 0's release store, whose L2 write-back
+// then carries all of them.
+__device__ __forceinline__ void publish(int* p, int value) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(p, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Helper workgroup h of column ic: after its column's step ib is published,
@@ -121,8 +146,7 @@ __device__ void sweep_helper(double* __restrict__ A, int lda, double* __restrict
       row0 = col + r;
     }
     wait_at_least(&progress[ic - 1], ib, errflag);
-    __syncthreads();
-    __threadfence();
+    acquire_waits();
     const int nb = imin(r, n - row0 + 1);
     const int istart = imax(s + 1, s + 1 + (ib - 1 - nq + ic) * r);
     const int nrows = row0 - istart;  // rows istart..row0-1 (1-based)
@@ -139,13 +163,9 @@ __device__ void sweep_helper(double* __restrict__ A, int lda, double* __restrict
                          row0 - 1, nb, hc, hs);
       }
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(mine, ib, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    publish(mine, ib);
   }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) __hip_atomic_store(mine, kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  publish(mine, kDone);
 }
 
 template <int RMAX>
@@ -159,7 +179,6 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
   __shared__ double xa[kMaxLdr], xb[kMaxLdr];
   __shared__ double red[2][kParts][128];
   extern __shared__ double Ml[];  // M of the current block: M(i, j) at Ml[j*(ldr+1) + i]
-  const int tid = threadIdx.x;
   if ((int)blockIdx.x >= nq) {  // helper workgroups follow the nq leaders
     const int hidx = blockIdx.x - nq;
     sweep_helper<RMAX>(A, lda, B, ldb, n, r, nq, s, CV, SV, ldr, progress, hprog, errflag,
@@ -183,22 +202,14 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
     if (ic > 1) {  // the previous column's leader and helpers are 2 steps ahead
       wait_at_least(&progress[ic - 2], ib + 2, errflag);
       for (int h = 0; h < nhelp; ++h) wait_at_least(&hprog[(ic - 2) * nhelp + h], ib + 2, errflag);
-      __syncthreads();
-      __threadfence();
+      acquire_waits();
     }
     chase_step<RMAX, kGenThreads>(A, lda, B, ldb, n, r, nq, s, ic, ib, col, row0, CU, SU, CV,
                                   SV, ldr, Mw, nhelp == 0, Bb, lc, ls, rc, rs, xa, xb,
                                   &red[0][0][0], Ml);
-    // ---- publish the step
-    __threadfence();
-    __syncthreads();
-    if (tid == 0)
-      __hip_atomic_store(&progress[ic - 1], ib, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    publish(&progress[ic - 1], ib);
   }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0)
-    __hip_atomic_store(&progress[ic - 1], kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  publish(&progress[ic - 1], kDone);
 }
 
 // ---------------------------------------------------------------------------
