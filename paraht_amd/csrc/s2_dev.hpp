@@ -37,9 +37,12 @@ __device__ __forceinline__ void rot_from_norm(double f, double g, double d,
 
 // Right rotations (rc, rs)[ig], ig = nb-1..1, of one step applied to row rr0
 // (0-based) of Mx on the step's columns R0..R0+nb-1.
+// A window row with fewer than RMAX columns (the last block of a sweep): kept
+// out of line so that its per-column tests and addresses do not take
+// registers in the kernels that inline the full-block path.
 template <int RMAX>
-__device__ __forceinline__ void rotate_row(double* __restrict__ Mx, int ld, int rr0, int R0,
-                                           int nb, const double* rc, const double* rs) {
+__device__ __noinline__ void rotate_row_partial(double* __restrict__ Mx, int ld, int rr0, int R0,
+                                                int nb, const double* rc, const double* rs) {
   double seg[RMAX];
 #pragma unroll
   for (int j = 0; j < RMAX; ++j) seg[j] = (j < nb) ? Mx[PHT_IDX(rr0, R0 + j, ld)] : 0.0;
@@ -59,6 +62,43 @@ __device__ __forceinline__ void rotate_row(double* __restrict__ Mx, int ld, int 
 #pragma unroll
   for (int j = 0; j < RMAX; ++j)
     if (j < nb) Mx[PHT_IDX(rr0, R0 + j, ld)] = seg[j];
+}
+
+// Row rr0 (0-based) of Mx, columns R0..R0+nb-1, times the right rotations
+// (rc, rs)[ig], ig = nb-1..1, each on columns (ig-1, ig).
+template <int RMAX>
+__device__ __forceinline__ void rotate_row(double* __restrict__ Mx, int ld, int rr0, int R0,
+                                           int nb, const double* rc, const double* rs) {
+  if (nb != RMAX) {
+    rotate_row_partial<RMAX>(Mx, ld, rr0, R0, nb, rc, rs);
+    return;
+  }
+  // the row's RMAX entries by a pointer walk (no per-column tests or address
+  // tables), all loads in flight together
+  double seg[RMAX];
+  double* p = Mx + PHT_IDX(rr0, R0, ld);
+#pragma unroll
+  for (int j = 0; j < RMAX; ++j) {
+    seg[j] = *p;
+    p += ld;
+  }
+  double carry = seg[RMAX - 1];
+#pragma unroll
+  for (int ig = RMAX - 1; ig >= 1; --ig) {
+    if (ig % 8 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds live coefficients
+    const double c = rc[ig], sn = rs[ig];
+    const double y = seg[ig - 1];
+    seg[ig] = c * carry + sn * y;
+    carry = c * y - sn * carry;
+  }
+  seg[0] = carry;
+  // a fresh walk for the stores: the load addresses are not kept alive
+  asm volatile("" : "+v"(p));
+#pragma unroll
+  for (int j = RMAX - 1; j >= 0; --j) {
+    p -= ld;
+    *p = seg[j];
+  }
 }
 
 // Right chain on the B block Bb (one wave, lane = block row): rotations
