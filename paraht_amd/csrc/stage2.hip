@@ -31,14 +31,16 @@ namespace pht {
 namespace {
 constexpr int kGenThreads = 256;
 constexpr int kTile = 128;
-// far updates in wavefront form (PARAHT_S2_WAVEFRONT=0: the sequential form)
-bool s2_wavefront() {
+// form of the far updates: 2 the split wavefront (two lanes per row or
+// column, the default), 1 the one-lane wavefront, 0 the sequential form
+// (PARAHT_S2_WAVEFRONT overrides)
+int s2_wavefront() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("PARAHT_S2_WAVEFRONT");
-    v = (e && *e == '0') ? 0 : 1;
+    v = (e && (*e == '0' || *e == '1')) ? *e - '0' : 2;
   }
-  return v == 1;
+  return v;
 }
 }  // namespace
 using namespace s2;
@@ -561,14 +563,20 @@ constexpr int wf_last(int R, int NQ, int e) {
   return l;
 }
 
-// Window entries whose first touch is step TAU+1 are read from the LDS tile
-// (one step ahead), entries whose last touch was step TAU are written back.
-template <int R, int NQ, int TAU, int E>
-__device__ __forceinline__ void wf_stream(double (&w)[R + NQ - 1], double (*tile)[65], int t) {
-  constexpr int NW = R + NQ - 1;
-  if constexpr (wf_first(R, NQ, E) == TAU + 1) w[E] = tile[E][t];
-  if constexpr (wf_last(R, NQ, E) == TAU) tile[E][t] = w[E];
-  if constexpr (E + 1 < NW) wf_stream<R, NQ, TAU, E + 1>(w, tile, t);
+// Window entry e of a thread lives at col[e * LDT] of the LDS tile.  After
+// step TAU, the entries whose last touch was step TAU are written back, then
+// the entries first touched at step TAU+1 are read (one step ahead).  The
+// writes come first: in the split form (two lanes per row, k_s2_rows_w2)
+// one lane's last write of an entry is the other lane's first read of it.
+template <int R, int NQ, int LDT, int TAU, int E>
+__device__ __forceinline__ void wf_store(double (&w)[R + NQ - 1], double* col) {
+  if constexpr (wf_last(R, NQ, E) == TAU) col[E * LDT] = w[E];
+  if constexpr (E + 1 < R + NQ - 1) wf_store<R, NQ, LDT, TAU, E + 1>(w, col);
+}
+template <int R, int NQ, int LDT, int TAU, int E>
+__device__ __forceinline__ void wf_load(double (&w)[R + NQ - 1], const double* col) {
+  if constexpr (wf_first(R, NQ, E) == TAU + 1) w[E] = col[E * LDT];
+  if constexpr (E + 1 < R + NQ - 1) wf_load<R, NQ, LDT, TAU, E + 1>(w, col);
 }
 
 // Coefficients of step TAU (sequence ICM's rotation ig = R-1-TAU+ICM) into
@@ -596,31 +604,77 @@ __device__ __forceinline__ void wf_rotations(double (&w)[R + NQ - 1], const doub
 }
 
 // Step TAU with the next step's coefficients and window entries read ahead.
-template <int R, int NQ, int TAU>
-__device__ __forceinline__ void wf_steps(double (&w)[R + NQ - 1], double (*tile)[65], int t,
+template <int R, int NQ, int LDT, int TAU>
+__device__ __forceinline__ void wf_steps(double (&w)[R + NQ - 1], double* col,
                                          const TapeBlock<R, NQ>& tb, int icm0,
                                          const double2 (&cs)[NQ]) {
   double2 nx[NQ];
   if constexpr (TAU + 1 < R + NQ - 2) wf_coeffs<R, NQ, TAU + 1, 0>(nx, tb);
   wf_rotations<R, NQ, TAU, 0>(w, cs, icm0);
-  wf_stream<R, NQ, TAU, 0>(w, tile, t);
+  wf_store<R, NQ, LDT, TAU, 0>(w, col);
+  wf_load<R, NQ, LDT, TAU, 0>(w, col);
   // one step's coefficients and streamed entries in flight at a time
   asm volatile("" ::: "memory");
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (TAU + 1 < R + NQ - 2) wf_steps<R, NQ, TAU + 1>(w, tile, t, tb, icm0, nx);
+  if constexpr (TAU + 1 < R + NQ - 2) wf_steps<R, NQ, LDT, TAU + 1>(w, col, tb, icm0, nx);
 }
 
-// Applies a block's NQ rotation sequences to column t of the LDS tile
-// (tile[e][t] = window entry e), in wavefront order; only the band of
+// Applies a block's NQ rotation sequences to one window held in an LDS
+// column (entry e at col[e * LDT]), in wavefront order; only the band of
 // entries the current steps touch is held in registers.
-template <int R, int NQ>
-__device__ __forceinline__ void wavefront_window(double (*tile)[65], int t,
-                                                 const TapeBlock<R, NQ>& tb, int icm0) {
+template <int R, int NQ, int LDT>
+__device__ __forceinline__ void wavefront_window(double* col, const TapeBlock<R, NQ>& tb,
+                                                 int icm0) {
   double w[R + NQ - 1];
   double2 cs[NQ];
   wf_coeffs<R, NQ, 0, 0>(cs, tb);
-  wf_stream<R, NQ, -1, 0>(w, tile, t);  // entries first touched at step 0
-  wf_steps<R, NQ, 0>(w, tile, t, tb, icm0, cs);
+  wf_load<R, NQ, LDT, -1, 0>(w, col);  // entries first touched at step 0
+  wf_steps<R, NQ, LDT, 0>(w, col, tb, icm0, cs);
+}
+
+// ---------------------------------------------------------------------------
+// Split form: two lanes per row (column).  Lane h applies the sequences
+// [h NQH, (h+1) NQH), NQH = NQ/2, at the same global steps as the one-lane
+// wavefront: at step tau its slot i holds rotation (h NQH + i,
+// ig = R-1-tau+h NQH+i), which touches entries 2i + 2h NQH + R-1-tau (-1).
+// With lane h's register e holding window entry e + (2h-1) NQH, both lanes
+// touch the same registers at every step, so they run one instruction
+// stream: a wavefront with R' = R + NQH, NQ' = NQH, whose slot i carries
+// rotation ig' - (1-h) NQH of the lane's sequence, the identity where that
+// falls outside [1, R-1].  Each lane executes NQH rotations per step instead
+// of NQ.  Lane 0's last write of an entry and lane 1's first read of it fall
+// after the same step (window entry x: R + 2 NQH - 3 - x), the write first.
+// The tile has NQH zero rows on either side (entries -NQH.. and NW..), which
+// only identity rotations touch.
+template <int R, int NQ, int NT>
+__device__ __forceinline__ void stage_tape_split(TapeBlock<R + NQ / 2, NQ / 2> (&tb)[2],
+                                                 const double* __restrict__ Ct,
+                                                 const double* __restrict__ St, int ib,
+                                                 double sgn) {
+  constexpr int NQH = NQ / 2, RP = R + NQH, LDR = R + NQ;
+  constexpr int TOT = 2 * NQH * RP;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  const size_t boff = (size_t)(ib - 1) * LDR * NQ;
+  double c[PER], sn[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * NT;
+    const int h = e / (NQH * RP), rem = e % (NQH * RP);
+    const int i = rem / RP, igp = rem % RP;
+    const int ig = igp - (1 - h) * NQH, icm = h * NQH + i;
+    const bool ok = e < TOT && ig >= 1 && ig <= R - 1;
+    const size_t ti = ok ? boff + (size_t)icm * LDR + (icm + ig - 1) : 0;
+    c[q] = ok ? Ct[ti] : 1.0;
+    sn[q] = ok ? St[ti] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * NT;
+    if (e < TOT) {
+      const int h = e / (NQH * RP), rem = e % (NQH * RP);
+      tb[h].cs[rem / RP][rem % RP] = make_double2(c[q], sgn * sn[q]);
+    }
+  }
 }
 
 // Rows (one per thread) in wavefront form.  Each thread's window lives in
@@ -690,9 +744,9 @@ __global__ __launch_bounds__(64) void k_s2_rows_w(
     // every sequence reaches every row of the wave (the common case): the
     // variant without per-rotation masks
     if (__all(icm0 == 0))
-      wavefront_window<R, NQ>(tile, t, tb, 0);
+      wavefront_window<R, NQ, 65>(&tile[0][t], tb, 0);
     else if (icm0 < NQ)
-      wavefront_window<R, NQ>(tile, t, tb, icm0);
+      wavefront_window<R, NQ, 65>(&tile[0][t], tb, icm0);
     if (valid)
       for (int k = 0; k < W; ++k) src[(size_t)k * ld] = tile[k][t];
     carry = true;
@@ -760,11 +814,163 @@ __global__ __launch_bounds__(64) void k_s2_cols_w(
     }
     __syncthreads();
     const bool act = (j < n) && (j + 1 > istop);
-    if (act) wavefront_window<R, NQ>(tile, t, tb, 0);
+    if (act) wavefront_window<R, NQ, 65>(&tile[0][t], tb, 0);
     __syncthreads();
     for (int c = 0; c < cmax; ++c) {
       if (j0 + c + 1 <= istop) continue;
       for (int k = t; k < W; k += 64) M[PHT_IDX(rw0 + k, j0 + c, ld)] = tile[k][c];
+    }
+    carry = true;
+  }
+}
+
+// Rows in the split wavefront form (two lanes per row, 32 rows per
+// workgroup); otherwise as k_s2_rows_w.
+template <int R, int NQ, bool AB>
+__global__ __launch_bounds__(64) void k_s2_rows_w2(
+    double* __restrict__ A, int lda, double* __restrict__ B, int ldb,
+    double* __restrict__ Q, int ldq, double* __restrict__ Z, int ldz, int n,
+    int s, int nblocks, const double* __restrict__ CU,
+    const double* __restrict__ SU, const double* __restrict__ CV,
+    const double* __restrict__ SV) {
+  constexpr int NQH = NQ / 2, RP = R + NQH, NW = R + NQ - 1, NP = NW + 2 * NQH, LDT = 33;
+  static_assert(NQ % 2 == 0, "two lanes take half of the sequences each");
+  __shared__ double tile[NP][LDT];  // row NQH + e: window entry e
+  __shared__ TapeBlock<RP, NQH> tb[2];
+  const int which = AB ? blockIdx.y : 2 + blockIdx.y;  // 0 A, 1 B, 2 Q, 3 Z
+  const int row_first = blockIdx.x * 32;
+  const int t = threadIdx.x, c = t >> 1, h = t & 1;
+  const int i = row_first + c;  // 0-based row
+  const bool valid = i < n;
+  double* M = which == 0 ? A : which == 1 ? B : which == 2 ? Q : Z;
+  const int ld = which == 0 ? lda : which == 1 ? ldb : which == 2 ? ldq : ldz;
+  const double* Ct = which == 2 ? CU : CV;
+  const double* St = which == 2 ? SU : SV;
+  const double sgn = which == 2 ? -1.0 : 1.0;
+  for (int e = t; e < NQH * LDT; e += 64) {
+    tile[e / LDT][e % LDT] = 0.0;
+    tile[NQH + NW + e / LDT][e % LDT] = 0.0;
+  }
+  bool carry = false;
+  for (int ib = nblocks; ib >= 1; --ib) {
+    const int cw0 = s + (ib - 1) * R;  // 0-based first window column
+    const int W = imin(NW, n - cw0);
+    if (W <= 0) {
+      carry = false;
+      continue;
+    }
+    if (AB && row_first + 1 >= imax(s + 1, s + 1 + (ib - 1) * R)) break;
+    __syncthreads();  // the previous block's coefficients are consumed
+    stage_tape_split<R, NQ, 64>(tb, Ct, St, ib, sgn);
+    double* src = M + PHT_IDX(valid ? i : 0, cw0, ld);
+    if (carry) {
+#pragma unroll
+      for (int k = 0; k < NQ - 1; k += 2)
+        if (k + h < NQ - 1) tile[NQH + R + k + h][c] = tile[NQH + k + h][c];
+    }
+    const int kf = carry ? R : NW;  // fresh entries [0, kf)
+#pragma unroll
+    for (int k0 = 0; k0 < NW; k0 += 32) {
+      double tmp[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = k0 + 2 * q + h;
+        tmp[q] = (valid && k < W && k < kf) ? src[(size_t)k * ld] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = k0 + 2 * q + h;
+        if (k < NW && k < kf) tile[NQH + k][c] = tmp[q];
+      }
+    }
+    __syncthreads();
+    int icm0 = valid ? 0 : NQ;
+    if (AB && valid) {
+      icm0 = NQ;
+#pragma unroll
+      for (int icm = NQ - 1; icm >= 0; --icm)
+        if (i + 1 < imax(s + 1, s + 1 + (ib - NQ + icm) * R)) icm0 = icm;
+    }
+    const int icm0h = icm0 - h * NQH;
+    double* col = &tile[2 * h * NQH][c];
+    if (__all(icm0h <= 0))
+      wavefront_window<RP, NQH, LDT>(col, tb[h], 0);
+    else if (icm0h < NQH)
+      wavefront_window<RP, NQH, LDT>(col, tb[h], icm0h);
+    if (valid)
+      for (int k = h; k < W; k += 2) src[(size_t)k * ld] = tile[NQH + k][c];
+    carry = true;
+  }
+}
+
+// Columns in the split wavefront form (two lanes per column, 32 columns per
+// workgroup); otherwise as k_s2_cols_w.
+template <int R, int NQ>
+__global__ __launch_bounds__(64) void k_s2_cols_w2(
+    double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
+    int s, int nblocks, const double* __restrict__ CU,
+    const double* __restrict__ SU) {
+  constexpr int NQH = NQ / 2, RP = R + NQH, NW = R + NQ - 1, NP = NW + 2 * NQH, LDT = 33;
+  static_assert(NQ % 2 == 0, "two lanes take half of the sequences each");
+  static_assert(NW <= 128, "two 64-row chunks per column");
+  __shared__ double tile[NP][LDT];  // row NQH + k: window row k
+  __shared__ TapeBlock<RP, NQH> tb[2];
+  const int which = blockIdx.y;
+  const int t = threadIdx.x, c = t >> 1, h = t & 1;
+  const int j0 = blockIdx.x * 32;
+  const int j = j0 + c;
+  double* M = which == 0 ? A : B;
+  const int ld = which == 0 ? lda : ldb;
+  const int cmax = imin(32, n - j0);
+  for (int e = t; e < NQH * LDT; e += 64) {
+    tile[e / LDT][e % LDT] = 0.0;
+    tile[NQH + NW + e / LDT][e % LDT] = 0.0;
+  }
+  bool carry = false;
+  for (int ib = nblocks; ib >= 1; --ib) {
+    const int istop = (which == 0) ? ((ib == 1) ? s + NQ - 1 : s + (ib - 2) * R + NQ)
+                                   : s + (ib - 1) * R + R + NQ - 1;  // 1-based
+    if (j0 + 32 <= istop) {  // no column of this workgroup takes part
+      carry = false;
+      continue;
+    }
+    const int rw0 = s + (ib - 1) * R;  // 0-based first row
+    const int W = imin(NW, n - rw0);
+    if (W <= 0) {
+      carry = false;
+      continue;
+    }
+    __syncthreads();  // the previous block's coefficients and tile are consumed
+    stage_tape_split<R, NQ, 64>(tb, CU, SU, ib, -1.0);
+    if (carry) {
+#pragma unroll
+      for (int k = 0; k < NQ - 1; k += 2)
+        if (k + h < NQ - 1) tile[NQH + R + k + h][c] = tile[NQH + k + h][c];
+    }
+    const int kf = carry ? R : NW;  // fresh rows [0, kf)
+    // 8 columns x 2 row chunks per group, all 16 loads in flight together
+#pragma unroll
+    for (int c0 = 0; c0 < 32; c0 += 8) {
+      double tmp[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int cc = c0 + (q >> 1), k = t + 64 * (q & 1);
+        tmp[q] = (cc < cmax && k < W && k < kf) ? M[PHT_IDX(rw0 + k, j0 + cc, ld)] : 0.0;
+      }
+      // the carry copies above precede these writes in the wave's order
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int cc = c0 + (q >> 1), k = t + 64 * (q & 1);
+        if (k < NW && k < kf) tile[NQH + k][cc] = tmp[q];
+      }
+    }
+    __syncthreads();
+    const bool act = (j < n) && (j + 1 > istop);
+    if (act) wavefront_window<RP, NQH, LDT>(&tile[2 * h * NQH][c], tb[h], 0);
+    __syncthreads();
+    for (int cc = 0; cc < cmax; ++cc) {
+      if (j0 + cc + 1 <= istop) continue;
+      for (int k = t; k < W; k += 64) M[PHT_IDX(rw0 + k, j0 + cc, ld)] = tile[NQH + k][cc];
     }
     carry = true;
   }
@@ -854,7 +1060,19 @@ hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
 #define PHT_ROWS(RR, NN)                                                        \
   if (P.r == RR && P.nq == NN) {                                                \
     const dim3 g((P.n + kRowsThreads - 1) / kRowsThreads, 2);                   \
-    if (wavefront_fits(RR, NN) && s2_wavefront()) {                             \
+    if (wavefront_fits(RR, NN) && s2_wavefront() == 2) {                        \
+      const dim3 g2((P.n + 31) / 32, 2);                                        \
+      if (part != kRowsQZ)                                                      \
+        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, true>), g2, dim3(64), 0, st, A, \
+                           lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
+                           P.nblocks, CU, SU, CV, SV);                          \
+      if (part != kRowsAB)                                                      \
+        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, false>), g2, dim3(64), 0, st, A, \
+                           lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
+                           P.nblocks, CU, SU, CV, SV);                          \
+      return hipGetLastError();                                                 \
+    }                                                                           \
+    if (wavefront_fits(RR, NN) && s2_wavefront() == 1) {                        \
       if (part != kRowsQZ)                                                      \
         hipLaunchKernelGGL((k_s2_rows_w<RR, NN, true>), g, dim3(64), 0, st, A,  \
                            lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
@@ -899,7 +1117,11 @@ hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
   }
 #define PHT_COLS(RR, NN)                                                     \
   if (P.r == RR && P.nq == NN) {                                             \
-    if (wavefront_fits(RR, NN) && s2_wavefront())                            \
+    if (wavefront_fits(RR, NN) && s2_wavefront() == 2)                       \
+      hipLaunchKernelGGL((k_s2_cols_w2<RR, NN>), dim3((P.n + 31) / 32, 2),   \
+                         dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,     \
+                         P.nblocks, CU, SU);                                 \
+    else if (wavefront_fits(RR, NN) && s2_wavefront() == 1)                  \
       hipLaunchKernelGGL((k_s2_cols_w<RR, NN>), dim3((P.n + 63) / 64, 2),    \
                          dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,     \
                          P.nblocks, CU, SU);                                 \
