@@ -163,6 +163,39 @@ __device__ __forceinline__ void left_chain_lds(double* p, int st, int nb, const 
   p[0] = carry;
 }
 
+// M of one block (ldr x ldr, column-major, identity for the first column
+// ic = 1) into LDS (ld ldr+1) by nw waves starting at wave w0: a wave per set
+// of columns, lanes along the rows (ldr <= kMaxLdr = 160 rows: three 64-row
+// chunks), 8 columns' loads in flight together.
+__device__ __forceinline__ void stage_m(double* Ml, const double* __restrict__ M, int ldr,
+                                        int ic, int w0, int nw) {
+  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) - w0;
+  const int ldl = ldr + 1;
+  for (int cm0 = wave; cm0 < ldr; cm0 += 8 * nw) {
+    double tmp[8][3];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int cm = cm0 + u * nw;
+#pragma unroll
+      for (int h = 0; h < 3; ++h) {
+        const int rw = lane + 64 * h;
+        tmp[u][h] = (cm < ldr && rw < ldr)
+                        ? ((ic > 1) ? M[(size_t)cm * ldr + rw] : (rw == cm ? 1.0 : 0.0))
+                        : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int cm = cm0 + u * nw;
+#pragma unroll
+      for (int h = 0; h < 3; ++h) {
+        const int rw = lane + 64 * h;
+        if (cm < ldr && rw < ldr) Ml[cm * ldl + rw] = tmp[u][h];
+      }
+    }
+  }
+}
+
 // One step (ic, ib) of a sweep starting at column s (rht2ht.f90:787-957), by
 // all NT threads of a workgroup:
 //   * M of block ib (the product of the deferred left rotations of columns
@@ -175,14 +208,19 @@ __device__ __forceinline__ void left_chain_lds(double* p, int st, int nb, const 
 //     and, with `above`, A and B rows istart..row0-1.
 // col, row0 are 1-based as in the reference.  LDS: Bb[RMAX][RMAX+1], lc, ls,
 // rc, rs [RMAX], xa, xb [kMaxLdr], red [2][NT/128][128], Ml [ldr][ldr+1].
+// With have_m, Ml already holds M of block ib.  With Mnext (a second LDS
+// buffer), waves 1.. stage M of block ib+1 into it while wave 0 runs the right
+// chain; the caller must have acquired that block's M (the previous column's
+// step ib+1 is complete).  Returns whether Mnext was filled.
 template <int RMAX, int NT>
-__device__ inline void chase_step(double* __restrict__ A, int lda, double* __restrict__ B,
+__device__ inline bool chase_step(double* __restrict__ A, int lda, double* __restrict__ B,
                                   int ldb, int n, int r, int nq, int s, int ic, int ib, int col,
                                   int row0, double* __restrict__ CU, double* __restrict__ SU,
                                   double* __restrict__ CV, double* __restrict__ SV, int ldr,
                                   double* __restrict__ Mw, bool above, double (*Bb)[RMAX + 1],
                                   double* lc, double* ls, double* rc, double* rs, double* xa,
-                                  double* xb, double* red, double* Ml) {
+                                  double* xb, double* red, double* Ml, bool have_m = false,
+                                  double* Mnext = nullptr) {
   constexpr int kParts = NT / 128;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -194,36 +232,10 @@ __device__ inline void chase_step(double* __restrict__ A, int lda, double* __res
   const int L = imin(ldr - 1, n - base);
   double* M = Mw + (size_t)(ib - 1) * msz;
 
-  // ---- M of this block into LDS (identity for the first column): one wave
-  //      per set of columns, lanes along the rows (ldr <= kMaxLdr = 160 rows:
-  //      three 64-row chunks), 8 columns' loads in flight together
-  {
-    constexpr int NWV = NT / 64;
-    for (int cm0 = wave; cm0 < ldr; cm0 += 8 * NWV) {
-      double tmp[8][3];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int cm = cm0 + u * NWV;
-#pragma unroll
-        for (int h = 0; h < 3; ++h) {
-          const int rw = lane + 64 * h;
-          tmp[u][h] = (cm < ldr && rw < ldr)
-                          ? ((ic > 1) ? M[(size_t)cm * ldr + rw] : (rw == cm ? 1.0 : 0.0))
-                          : 0.0;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int cm = cm0 + u * NWV;
-#pragma unroll
-        for (int h = 0; h < 3; ++h) {
-          const int rw = lane + 64 * h;
-          if (cm < ldr && rw < ldr) Ml[cm * ldl + rw] = tmp[u][h];
-        }
-      }
-    }
-  }
+  // ---- M of this block into LDS (unless the previous step staged it)
+  if (!have_m) stage_m(Ml, M, ldr, ic, 0, NT / 64);
   __syncthreads();
+  bool staged_next = false;
 
   // ---- deferred left rotations of the sweep's earlier columns:
   //      A(:, col) and B(:, row0+r-1), local rows 1..L of block ib.
@@ -340,7 +352,9 @@ __device__ inline void chase_step(double* __restrict__ A, int lda, double* __res
         const int k = e % RMAX, cm = e / RMAX;
         if (k < nb && cm < ldr) M[(ic - 1) + k + (size_t)cm * ldr] = Ml[cm * ldl + (ic - 1) + k];
       }
+      if (NT > 64 && Mnext != nullptr) stage_m(Mnext, M + msz, ldr, ic, 1, NT / 64 - 1);
     }
+    staged_next = NT > 64 && Mnext != nullptr;
 
     // ---- right chain on the B block (rht2ht.f90:843-851,927-935):
     //      wave 0, lane = block row; the pivot values are read from lane ig
@@ -371,6 +385,7 @@ __device__ inline void chase_step(double* __restrict__ A, int lda, double* __res
       }
     }
   }
+  return staged_next;
 }
 
 }  // namespace s2

@@ -31,16 +31,15 @@ namespace pht {
 namespace {
 constexpr int kGenThreads = 256;
 constexpr int kTile = 128;
-// form of the far updates: 2 the split wavefront (two lanes per row or
-// column, the default), 1 the one-lane wavefront, 0 the sequential form
-// (PARAHT_S2_WAVEFRONT overrides)
-int s2_wavefront() {
+// far updates in the split wavefront form (PARAHT_S2_WAVEFRONT=0: the
+// sequential form)
+bool s2_wavefront() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("PARAHT_S2_WAVEFRONT");
-    v = (e && (*e == '0' || *e == '1')) ? *e - '0' : 2;
+    v = (e && *e == '0') ? 0 : 1;
   }
-  return v;
+  return v == 1;
 }
 }  // namespace
 using namespace s2;
@@ -175,12 +174,15 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
     int r, int nq, int s, double* __restrict__ CU, double* __restrict__ SU,
     double* __restrict__ CV, double* __restrict__ SV, int ldr,
-    double* __restrict__ Mw, int* progress, int* errflag, int nhelp, int* hprog) {
+    double* __restrict__ Mw, int* progress, int* errflag, int nhelp, int* hprog,
+    int mbufs) {
   __shared__ double Bb[RMAX][RMAX + 1];
   __shared__ double lc[RMAX], ls[RMAX], rc[RMAX], rs[RMAX];
   __shared__ double xa[kMaxLdr], xb[kMaxLdr];
   __shared__ double red[2][kParts][128];
-  extern __shared__ double Ml[];  // M of the current block: M(i, j) at Ml[j*(ldr+1) + i]
+  // M of the current block: M(i, j) at Ml[j*(ldr+1) + i]; with mbufs = 2 a
+  // second buffer receives the next block's M during the current step
+  extern __shared__ double Ml[];
   if ((int)blockIdx.x >= nq) {  // helper workgroups follow the nq leaders
     const int hidx = blockIdx.x - nq;
     sweep_helper<RMAX>(A, lda, B, ldb, n, r, nq, s, CV, SV, ldr, progress, hprog, errflag,
@@ -189,6 +191,8 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
   }
   const int ic = blockIdx.x + 1;
   const int jc = s + ic - 1;
+  const size_t mstride = (size_t)ldr * (ldr + 1);
+  bool have_m = false;
 
   for (int ib = 1;; ++ib) {
     int col, row0;
@@ -206,9 +210,13 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
       for (int h = 0; h < nhelp; ++h) wait_at_least(&hprog[(ic - 2) * nhelp + h], ib + 2, errflag);
       acquire_waits();
     }
-    chase_step<RMAX, kGenThreads>(A, lda, B, ldb, n, r, nq, s, ic, ib, col, row0, CU, SU, CV,
-                                  SV, ldr, Mw, nhelp == 0, Bb, lc, ls, rc, rs, xa, xb,
-                                  &red[0][0][0], Ml);
+    // the next block exists iff its column jc + 1 + (ib-1) r is inside the matrix
+    const bool next = jc + 1 + (ib - 1) * r <= n;
+    double* Mcur = Ml + (mbufs == 2 ? (size_t)(ib & 1) * mstride : 0);
+    double* Mnxt = (mbufs == 2 && next) ? Ml + (size_t)((ib + 1) & 1) * mstride : nullptr;
+    have_m = chase_step<RMAX, kGenThreads>(A, lda, B, ldb, n, r, nq, s, ic, ib, col, row0, CU,
+                                           SU, CV, SV, ldr, Mw, nhelp == 0, Bb, lc, ls, rc, rs,
+                                           xa, xb, &red[0][0][0], Mcur, have_m, Mnxt);
     publish(&progress[ic - 1], ib);
   }
   publish(&progress[ic - 1], kDone);
@@ -645,7 +653,10 @@ __device__ __forceinline__ void wavefront_window(double* col, const TapeBlock<R,
 // of NQ.  Lane 0's last write of an entry and lane 1's first read of it fall
 // after the same step (window entry x: R + 2 NQH - 3 - x), the write first.
 // The tile has NQH zero rows on either side (entries -NQH.. and NW..), which
-// only identity rotations touch.
+// only identity rotations touch.  (The one-lane form is this with all NQ
+// sequences in one lane; two lanes halve the rotations per lane and step and
+// double the waves, which the far updates need: a wave per 64 rows leaves
+// three of a CU's four SIMDs idle.)
 template <int R, int NQ, int NT>
 __device__ __forceinline__ void stage_tape_split(TapeBlock<R + NQ / 2, NQ / 2> (&tb)[2],
                                                  const double* __restrict__ Ct,
@@ -677,155 +688,12 @@ __device__ __forceinline__ void stage_tape_split(TapeBlock<R + NQ / 2, NQ / 2> (
   }
 }
 
-// Rows (one per thread) in wavefront form.  Each thread's window lives in
-// its own column of the LDS tile; consecutive blocks share NQ-1 entries
-// (block ib-1's entries [R, R+NQ-1) are block ib's [0, NQ-1)), which are
-// carried in the tile, so each later block loads R fresh entries.  AB: rows
-// of A (blockIdx.y = 0) and B (1) above the generation window; otherwise all
-// rows of Q (0) and Z (1).
-template <int R, int NQ, bool AB>
-__global__ __launch_bounds__(64) void k_s2_rows_w(
-    double* __restrict__ A, int lda, double* __restrict__ B, int ldb,
-    double* __restrict__ Q, int ldq, double* __restrict__ Z, int ldz, int n,
-    int s, int nblocks, const double* __restrict__ CU,
-    const double* __restrict__ SU, const double* __restrict__ CV,
-    const double* __restrict__ SV) {
-  constexpr int NW = R + NQ - 1;
-  __shared__ double tile[NW][65];
-  __shared__ TapeBlock<R, NQ> tb;
-  const int which = AB ? blockIdx.y : 2 + blockIdx.y;  // 0 A, 1 B, 2 Q, 3 Z
-  const int row_first = blockIdx.x * 64;
-  const int t = threadIdx.x;
-  const int i = row_first + t;  // 0-based row
-  const bool valid = i < n;
-  double* M = which == 0 ? A : which == 1 ? B : which == 2 ? Q : Z;
-  const int ld = which == 0 ? lda : which == 1 ? ldb : which == 2 ? ldq : ldz;
-  const double* Ct = which == 2 ? CU : CV;
-  const double* St = which == 2 ? SU : SV;
-  const double sgn = which == 2 ? -1.0 : 1.0;
-  bool carry = false;
-  for (int ib = nblocks; ib >= 1; --ib) {
-    const int cw0 = s + (ib - 1) * R;  // 0-based first window column
-    const int W = imin(NW, n - cw0);
-    if (W <= 0) {
-      carry = false;
-      continue;
-    }
-    // A, B: rows at or below istart(nq, ib) are the generation's, and
-    // istart only falls with ib
-    if (AB && row_first + 1 >= imax(s + 1, s + 1 + (ib - 1) * R)) break;
-    __syncthreads();  // the previous block's coefficients are consumed
-    stage_tape<R, NQ, 64>(tb, Ct, St, ib, sgn);
-    double* src = M + PHT_IDX(valid ? i : 0, cw0, ld);
-    if (carry) {
-#pragma unroll
-      for (int k = 0; k < NQ - 1; ++k) tile[R + k][t] = tile[k][t];
-    }
-    const int kf = carry ? R : NW;  // fresh entries [0, kf)
-#pragma unroll
-    for (int k0 = 0; k0 < NW; k0 += 16) {
-      double tmp[16];
-#pragma unroll
-      for (int k = 0; k < 16; ++k)
-        tmp[k] = (valid && k0 + k < W && k0 + k < kf) ? src[(size_t)(k0 + k) * ld] : 0.0;
-#pragma unroll
-      for (int k = 0; k < 16; ++k)
-        if (k0 + k < NW && k0 + k < kf) tile[k0 + k][t] = tmp[k];
-    }
-    __syncthreads();
-    int icm0 = valid ? 0 : NQ;
-    if (AB && valid) {
-      // sequence icm reaches this row iff i+1 < istart(icm), which rises with icm
-      icm0 = NQ;
-#pragma unroll
-      for (int icm = NQ - 1; icm >= 0; --icm)
-        if (i + 1 < imax(s + 1, s + 1 + (ib - NQ + icm) * R)) icm0 = icm;
-    }
-    // every sequence reaches every row of the wave (the common case): the
-    // variant without per-rotation masks
-    if (__all(icm0 == 0))
-      wavefront_window<R, NQ, 65>(&tile[0][t], tb, 0);
-    else if (icm0 < NQ)
-      wavefront_window<R, NQ, 65>(&tile[0][t], tb, icm0);
-    if (valid)
-      for (int k = 0; k < W; ++k) src[(size_t)k * ld] = tile[k][t];
-    carry = true;
-  }
-}
-
-// Columns (one per thread) of A (which = 0) or B (1) right of the generation
-// window, in wavefront form (left rotations in row form: the transposed
-// coefficients).  Global accesses run along the columns through the tile;
-// the tile carries the shared entries between blocks as in k_s2_rows_w.
-template <int R, int NQ>
-__global__ __launch_bounds__(64) void k_s2_cols_w(
-    double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
-    int s, int nblocks, const double* __restrict__ CU,
-    const double* __restrict__ SU) {
-  constexpr int NW = R + NQ - 1;
-  static_assert(NW <= 128, "two 64-row chunks per column");
-  __shared__ double tile[NW][65];
-  __shared__ TapeBlock<R, NQ> tb;
-  const int which = blockIdx.y;
-  const int t = threadIdx.x;
-  const int j0 = blockIdx.x * 64;
-  const int j = j0 + t;
-  double* M = which == 0 ? A : B;
-  const int ld = which == 0 ? lda : ldb;
-  const int cmax = imin(64, n - j0);
-  bool carry = false;
-  for (int ib = nblocks; ib >= 1; --ib) {
-    const int istop = (which == 0) ? ((ib == 1) ? s + NQ - 1 : s + (ib - 2) * R + NQ)
-                                   : s + (ib - 1) * R + R + NQ - 1;  // 1-based
-    if (j0 + 64 <= istop) {  // no column of this workgroup takes part
-      carry = false;
-      continue;
-    }
-    const int rw0 = s + (ib - 1) * R;  // 0-based first row
-    const int W = imin(NW, n - rw0);
-    if (W <= 0) {
-      carry = false;
-      continue;
-    }
-    __syncthreads();  // the previous block's coefficients and tile are consumed
-    stage_tape<R, NQ, 64>(tb, CU, SU, ib, -1.0);
-    if (carry) {
-#pragma unroll
-      for (int k = 0; k < NQ - 1; ++k) tile[R + k][t] = tile[k][t];
-    }
-    const int kf = carry ? R : NW;  // fresh rows [0, kf)
-    // 8 columns x 2 row chunks per group, all 16 loads in flight together
-#pragma unroll
-    for (int c0 = 0; c0 < 64; c0 += 8) {
-      double tmp[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int c = c0 + (q >> 1), k = t + 64 * (q & 1);
-        tmp[q] = (c < cmax && k < W && k < kf) ? M[PHT_IDX(rw0 + k, j0 + c, ld)] : 0.0;
-      }
-      // the carry copy above reads rows [0, NQ-1) of this thread's column
-      // only, so the fresh rows may overwrite other columns' rows now
-      if (c0 == 0) __syncthreads();
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int c = c0 + (q >> 1), k = t + 64 * (q & 1);
-        if (k < NW && k < kf) tile[k][c] = tmp[q];
-      }
-    }
-    __syncthreads();
-    const bool act = (j < n) && (j + 1 > istop);
-    if (act) wavefront_window<R, NQ, 65>(&tile[0][t], tb, 0);
-    __syncthreads();
-    for (int c = 0; c < cmax; ++c) {
-      if (j0 + c + 1 <= istop) continue;
-      for (int k = t; k < W; k += 64) M[PHT_IDX(rw0 + k, j0 + c, ld)] = tile[k][c];
-    }
-    carry = true;
-  }
-}
-
-// Rows in the split wavefront form (two lanes per row, 32 rows per
-// workgroup); otherwise as k_s2_rows_w.
+// Rows in the split wavefront form, two lanes per row and 32 rows per
+// workgroup.  Each row's window lives in its own column of the LDS tile;
+// consecutive blocks share NQ-1 entries (block ib-1's entries [R, R+NQ-1)
+// are block ib's [0, NQ-1)), which are carried in the tile, so each later
+// block loads R fresh entries.  AB: rows of A (blockIdx.y = 0) and B (1)
+// above the generation window; otherwise all rows of Q (0) and Z (1).
 template <int R, int NQ, bool AB>
 __global__ __launch_bounds__(64) void k_s2_rows_w2(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb,
@@ -903,8 +771,11 @@ __global__ __launch_bounds__(64) void k_s2_rows_w2(
   }
 }
 
-// Columns in the split wavefront form (two lanes per column, 32 columns per
-// workgroup); otherwise as k_s2_cols_w.
+// Columns of A (which = 0) or B (1) right of the generation window in the
+// split wavefront form, two lanes per column and 32 columns per workgroup
+// (left rotations in row form: the transposed coefficients).  Global
+// accesses run along the columns through the tile; the tile carries the
+// shared entries between blocks as in k_s2_rows_w2.
 template <int R, int NQ>
 __global__ __launch_bounds__(64) void k_s2_cols_w2(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
@@ -1013,8 +884,12 @@ hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
                        const S2Params& P, double* CU, double* SU, double* CV,
                        double* SV) {
   const dim3 g(P.nq * (1 + P.nhelp)), b(kGenThreads);
-  const size_t lds = s2_sweep_lds(P.r, P.nq);
-  if (lds == 0) return hipErrorInvalidValue;
+  const size_t one = s2_sweep_lds(P.r, P.nq);
+  if (one == 0) return hipErrorInvalidValue;
+  // two M buffers (the next block's M staged during the current step) when
+  // they fit beside the static LDS
+  const int mbufs = sweep_static_lds(sweep_rmax(P.r)) + 2 * one <= kLdsBudget ? 2 : 1;
+  const size_t lds = mbufs * one;
 #define PHT_GEN(R)                                                                \
   do {                                                                            \
     static bool attr = false;                                                     \
@@ -1027,7 +902,7 @@ hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
     }                                                                             \
     hipLaunchKernelGGL(k_s2_sweep<R>, g, b, lds, st, A, lda, B, ldb, P.n, P.r, P.nq, \
                        P.jblock, CU, SU, CV, SV, P.ldr, P.M, P.progress, P.errflag, \
-                       P.nhelp, P.hprog);                                          \
+                       P.nhelp, P.hprog, mbufs);                                   \
   } while (0)
   if (P.r <= 8)
     PHT_GEN(8);
@@ -1060,7 +935,7 @@ hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
 #define PHT_ROWS(RR, NN)                                                        \
   if (P.r == RR && P.nq == NN) {                                                \
     const dim3 g((P.n + kRowsThreads - 1) / kRowsThreads, 2);                   \
-    if (wavefront_fits(RR, NN) && s2_wavefront() == 2) {                        \
+    if (wavefront_fits(RR, NN) && s2_wavefront()) {                             \
       const dim3 g2((P.n + 31) / 32, 2);                                        \
       if (part != kRowsQZ)                                                      \
         hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, true>), g2, dim3(64), 0, st, A, \
@@ -1068,17 +943,6 @@ hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
                            P.nblocks, CU, SU, CV, SV);                          \
       if (part != kRowsAB)                                                      \
         hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, false>), g2, dim3(64), 0, st, A, \
-                           lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
-                           P.nblocks, CU, SU, CV, SV);                          \
-      return hipGetLastError();                                                 \
-    }                                                                           \
-    if (wavefront_fits(RR, NN) && s2_wavefront() == 1) {                        \
-      if (part != kRowsQZ)                                                      \
-        hipLaunchKernelGGL((k_s2_rows_w<RR, NN, true>), g, dim3(64), 0, st, A,  \
-                           lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
-                           P.nblocks, CU, SU, CV, SV);                          \
-      if (part != kRowsAB)                                                      \
-        hipLaunchKernelGGL((k_s2_rows_w<RR, NN, false>), g, dim3(64), 0, st, A, \
                            lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
                            P.nblocks, CU, SU, CV, SV);                          \
       return hipGetLastError();                                                 \
@@ -1117,12 +981,8 @@ hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
   }
 #define PHT_COLS(RR, NN)                                                     \
   if (P.r == RR && P.nq == NN) {                                             \
-    if (wavefront_fits(RR, NN) && s2_wavefront() == 2)                       \
+    if (wavefront_fits(RR, NN) && s2_wavefront())                            \
       hipLaunchKernelGGL((k_s2_cols_w2<RR, NN>), dim3((P.n + 31) / 32, 2),   \
-                         dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,     \
-                         P.nblocks, CU, SU);                                 \
-    else if (wavefront_fits(RR, NN) && s2_wavefront() == 1)                  \
-      hipLaunchKernelGGL((k_s2_cols_w<RR, NN>), dim3((P.n + 63) / 64, 2),    \
                          dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,     \
                          P.nblocks, CU, SU);                                 \
     else                                                                     \
