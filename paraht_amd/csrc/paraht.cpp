@@ -313,16 +313,19 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   std::vector<ApplyBlock> BR = RR;
   for (auto& b : BR) b.hi = b.start + b.len;
 
+  // the panel tapes are double-buffered: panel k's Q and Z updates run on the
+  // second stream while panel k+1 is generated into the other buffers
   const size_t vsz = (size_t)nbmax * P * nb, tsz = (size_t)nbmax * nb * nb;
   const size_t scr = (size_t)P * P + (size_t)nb * P;
   const size_t dsz = 5 * nd * sizeof(ApplyBlock);
-  size_t bytes = (2 * vsz + 2 * tsz + scr) * sizeof(double) + dsz + 256;
+  size_t bytes = (4 * vsz + 4 * tsz + scr) * sizeof(double) + dsz + 256;
   if (int e = ensure_ws(c, bytes)) return e;
-  double* VL = (double*)c->ws;
-  double* VR = VL + vsz;
-  double* TL = VR + vsz;
-  double* TR = TL + tsz;
-  double* scratch = TR + tsz;
+  if (int e = ensure_aux(c)) return e;
+  double* VLb[2] = {(double*)c->ws, (double*)c->ws + vsz};
+  double* VRb[2] = {VLb[1] + vsz, VLb[1] + 2 * vsz};
+  double* TLb[2] = {VRb[1] + vsz, VRb[1] + vsz + tsz};
+  double* TRb[2] = {TLb[1] + tsz, TLb[1] + 2 * tsz};
+  double* scratch = TRb[1] + tsz;
   ApplyBlock* dD = (ApplyBlock*)(((uintptr_t)(scratch + scr) + 255) & ~(uintptr_t)255);
   ApplyBlock* dBL = dD;
   ApplyBlock* dAL = dBL + nd;
@@ -342,7 +345,15 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   const bool fast_gen = nb <= 64 && P <= 128 && getenv("PARAHT_GENERIC_S1") == nullptr;
   Timer tm(c, tc);
   std::vector<int> marks;
+  int kp = 0;
   for (const PanelInfo& pi : panels) {
+    const int buf = kp & 1;
+    double* VL = VLb[buf];
+    double* VR = VRb[buf];
+    double* TL = TLb[buf];
+    double* TR = TRb[buf];
+    // panel kp-2's Q and Z updates (second stream) have read these tapes
+    if (kp >= 2) CK(hipStreamWaitEvent(st, c->ev_free[buf], 0));
     const int m0 = tm.mark();
     double qf = 0.0;
     for (int bl = 0; bl < pi.nblocks; ++bl) qf += qr_flops(BL[pi.first + bl].len, BL[pi.first + bl].k);
@@ -361,9 +372,6 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     PROF(PARAHT_K_WY_APPLY, wy_flops(AL, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kLeftT, A, lda, VL, P, TL, nb, dAL + pi.first,
                            pi.nblocks, nb, P, pi.jb0 + nb, n));
-    PROF(PARAHT_K_WY_APPLY, wy_flops(QL, pi.first, pi.nblocks), 0.0,
-         pht::apply_blocks(st, pht::kRightN, Q, ldq, VL, P, TL, nb, dQL + pi.first,
-                           pi.nblocks, nb, P, 0, n));
     const int m2 = tm.mark();
     for (int bl = 0; bl < pi.nblocks; ++bl) {
       const ApplyBlock& b = BR[pi.first + bl];
@@ -386,12 +394,23 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     PROF(PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
                            pi.nblocks, nb, P, 0, n));
-    PROF(PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
-         pht::apply_blocks(st, pht::kRightN, Z, ldz, VR, P, TR, nb, dRR + pi.first,
-                           pi.nblocks, nb, P, 0, n));
     const int m4 = tm.mark();
+    // Q and Z are written by these updates only: they leave the chain of
+    // panel generations for the second stream (in panel order there)
+    CK(hipEventRecord(c->ev_tape, st));
+    CK(hipStreamWaitEvent(c->aux, c->ev_tape, 0));
+    PROF_ON(c->aux, PARAHT_K_WY_APPLY, wy_flops(QL, pi.first, pi.nblocks), 0.0,
+            pht::apply_blocks(c->aux, pht::kRightN, Q, ldq, VL, P, TL, nb, dQL + pi.first,
+                              pi.nblocks, nb, P, 0, n));
+    PROF_ON(c->aux, PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
+            pht::apply_blocks(c->aux, pht::kRightN, Z, ldz, VR, P, TR, nb, dRR + pi.first,
+                              pi.nblocks, nb, P, 0, n));
+    CK(hipEventRecord(c->ev_free[buf], c->aux));
     marks.insert(marks.end(), {m0, m1, m2, m3, m4});
+    ++kp;
   }
+  // join: the last panel's Q and Z updates (the second stream runs in order)
+  if (kp > 0) CK(hipStreamWaitEvent(st, c->ev_free[(kp - 1) & 1], 0));
   if (tc) {
     CK(hipStreamSynchronize(st));
     double gl = 0, gr = 0, mul = 0;
