@@ -39,6 +39,8 @@ struct paraht_ctx {
   hipStream_t aux = nullptr;
   hipEvent_t ev_tape = nullptr;   // a sweep's tape is complete (stream -> aux)
   hipEvent_t ev_free[2] = {nullptr, nullptr};  // aux is done with tape buffer b
+  hipEvent_t ev_left = nullptr;   // a panel's left tapes are complete (stream -> aux)
+  hipEvent_t ev_aleft = nullptr;  // aux has applied a panel's left update to A
   void* ws = nullptr;
   size_t ws_bytes = 0;
   double times[8] = {0};
@@ -179,6 +181,8 @@ int ensure_aux(paraht_ctx* c) {
   CK(hipEventCreateWithFlags(&c->ev_tape, hipEventDisableTiming));
   CK(hipEventCreateWithFlags(&c->ev_free[0], hipEventDisableTiming));
   CK(hipEventCreateWithFlags(&c->ev_free[1], hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->ev_left, hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->ev_aleft, hipEventDisableTiming));
   CK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
   return 0;
 }
@@ -366,12 +370,17 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
            pht::s1_panel_qr(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
                             (long long)P * nb, TL, nb, (long long)nb * nb));
     const int m1 = tm.mark();
+    // the RQ chain reads only B: the left update of A runs beside it on the
+    // second stream, and the right update of A waits for it
+    CK(hipEventRecord(c->ev_left, st));
+    CK(hipStreamWaitEvent(c->aux, c->ev_left, 0));
+    PROF_ON(c->aux, PARAHT_K_WY_APPLY, wy_flops(AL, pi.first, pi.nblocks), 0.0,
+            pht::apply_blocks(c->aux, pht::kLeftT, A, lda, VL, P, TL, nb, dAL + pi.first,
+                              pi.nblocks, nb, P, pi.jb0 + nb, n));
+    CK(hipEventRecord(c->ev_aleft, c->aux));
     PROF(PARAHT_K_WY_APPLY, wy_flops(BL, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kLeftT, B, ldb, VL, P, TL, nb, dBL + pi.first,
                            pi.nblocks, nb, P, pi.i0top, n));
-    PROF(PARAHT_K_WY_APPLY, wy_flops(AL, pi.first, pi.nblocks), 0.0,
-         pht::apply_blocks(st, pht::kLeftT, A, lda, VL, P, TL, nb, dAL + pi.first,
-                           pi.nblocks, nb, P, pi.jb0 + nb, n));
     const int m2 = tm.mark();
     for (int bl = 0; bl < pi.nblocks; ++bl) {
       const ApplyBlock& b = BR[pi.first + bl];
@@ -391,6 +400,7 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
            pht::zero_below(st, B, ldb, b.start, b.len, b.len > nb ? nb : b.len));
     }
     const int m3 = tm.mark();
+    CK(hipStreamWaitEvent(st, c->ev_aleft, 0));
     PROF(PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
          pht::apply_blocks(st, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
                            pi.nblocks, nb, P, 0, n));
@@ -823,6 +833,8 @@ int paraht_ctx_destroy(paraht_ctx* ctx) {
   if (ctx->own) (void)hipStreamDestroy(ctx->own);
   if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
   if (ctx->ev_tape) (void)hipEventDestroy(ctx->ev_tape);
+  if (ctx->ev_left) (void)hipEventDestroy(ctx->ev_left);
+  if (ctx->ev_aleft) (void)hipEventDestroy(ctx->ev_aleft);
   for (hipEvent_t e : ctx->ev_free)
     if (e) (void)hipEventDestroy(e);
   delete ctx;
