@@ -574,7 +574,7 @@ constexpr int wf_last(int R, int NQ, int e) {
 // Window entry e of a thread lives at col[e * LDT] of the LDS tile.  After
 // step TAU, the entries whose last touch was step TAU are written back, then
 // the entries first touched at step TAU+1 are read (one step ahead).  The
-// writes come first: in the split form (two lanes per row, k_s2_rows_w2)
+// writes come first: in the split form (several lanes per row, k_s2_rows_w2)
 // one lane's last write of an entry is the other lane's first read of it.
 template <int R, int NQ, int LDT, int TAU, int E>
 __device__ __forceinline__ void wf_store(double (&w)[R + NQ - 1], double* col) {
@@ -641,29 +641,27 @@ __device__ __forceinline__ void wavefront_window(double* col, const TapeBlock<R,
 }
 
 // ---------------------------------------------------------------------------
-// Split form: two lanes per row (column).  Lane h applies the sequences
-// [h NQH, (h+1) NQH), NQH = NQ/2, at the same global steps as the one-lane
-// wavefront: at step tau its slot i holds rotation (h NQH + i,
-// ig = R-1-tau+h NQH+i), which touches entries 2i + 2h NQH + R-1-tau (-1).
-// With lane h's register e holding window entry e + (2h-1) NQH, both lanes
-// touch the same registers at every step, so they run one instruction
-// stream: a wavefront with R' = R + NQH, NQ' = NQH, whose slot i carries
-// rotation ig' - (1-h) NQH of the lane's sequence, the identity where that
-// falls outside [1, R-1].  Each lane executes NQH rotations per step instead
-// of NQ.  Lane 0's last write of an entry and lane 1's first read of it fall
-// after the same step (window entry x: R + 2 NQH - 3 - x), the write first.
-// The tile has NQH zero rows on either side (entries -NQH.. and NW..), which
-// only identity rotations touch.  (The one-lane form is this with all NQ
-// sequences in one lane; two lanes halve the rotations per lane and step and
-// double the waves, which the far updates need: a wave per 64 rows leaves
-// three of a CU's four SIMDs idle.)
-template <int R, int NQ, int NT>
-__device__ __forceinline__ void stage_tape_split(TapeBlock<R + NQ / 2, NQ / 2> (&tb)[2],
-                                                 const double* __restrict__ Ct,
-                                                 const double* __restrict__ St, int ib,
-                                                 double sgn) {
-  constexpr int NQH = NQ / 2, RP = R + NQH, LDR = R + NQ;
-  constexpr int TOT = 2 * NQH * RP;
+// Split form: L lanes per row (column).  Lane h applies the sequences
+// [h NQL, (h+1) NQL), NQL = NQ/L, at the same global steps as the one-lane
+// wavefront: at step tau its slot i holds rotation (h NQL + i,
+// ig = R-1-tau+h NQL+i), which touches entries 2i + 2h NQL + R-1-tau (-1).
+// With lane h's register e holding window entry e + (2h-(L-1)) NQL, all
+// lanes touch the same registers at every step, so they run one instruction
+// stream: a wavefront with R' = R + (L-1) NQL, NQ' = NQL, whose slot i
+// carries rotation ig' - (L-1-h) NQL of the lane's sequence, the identity
+// where that falls outside [1, R-1].  Each lane executes NQL rotations per
+// step instead of NQ.  Lane h's last write of an entry and lane h+1's first
+// read of it fall after the same step, the write first.  The tile has
+// (L-1) NQL zero rows on either side, which only identity rotations touch.
+// (One lane per row gives one wave per 64 rows and leaves three of a CU's
+// four SIMDs idle; L lanes cut the rotations per lane and step by L and run
+// L times as many waves.)
+template <int R, int NQ, int L, int NT>
+__device__ __forceinline__ void stage_tape_split(
+    TapeBlock<R + (L - 1) * (NQ / L), NQ / L> (&tb)[L], const double* __restrict__ Ct,
+    const double* __restrict__ St, int ib, double sgn) {
+  constexpr int NQH = NQ / L, RP = R + (L - 1) * NQH, LDR = R + NQ;
+  constexpr int TOT = L * NQH * RP;
   constexpr int PER = (TOT + NT - 1) / NT;
   const size_t boff = (size_t)(ib - 1) * LDR * NQ;
   double c[PER], sn[PER];
@@ -672,7 +670,7 @@ __device__ __forceinline__ void stage_tape_split(TapeBlock<R + NQ / 2, NQ / 2> (
     const int e = threadIdx.x + q * NT;
     const int h = e / (NQH * RP), rem = e % (NQH * RP);
     const int i = rem / RP, igp = rem % RP;
-    const int ig = igp - (1 - h) * NQH, icm = h * NQH + i;
+    const int ig = igp - (L - 1 - h) * NQH, icm = h * NQH + i;
     const bool ok = e < TOT && ig >= 1 && ig <= R - 1;
     const size_t ti = ok ? boff + (size_t)icm * LDR + (icm + ig - 1) : 0;
     c[q] = ok ? Ct[ti] : 1.0;
@@ -688,26 +686,27 @@ __device__ __forceinline__ void stage_tape_split(TapeBlock<R + NQ / 2, NQ / 2> (
   }
 }
 
-// Rows in the split wavefront form, two lanes per row and 32 rows per
+// Rows in the split wavefront form, L lanes per row and 64/L rows per
 // workgroup.  Each row's window lives in its own column of the LDS tile;
 // consecutive blocks share NQ-1 entries (block ib-1's entries [R, R+NQ-1)
 // are block ib's [0, NQ-1)), which are carried in the tile, so each later
 // block loads R fresh entries.  AB: rows of A (blockIdx.y = 0) and B (1)
 // above the generation window; otherwise all rows of Q (0) and Z (1).
-template <int R, int NQ, bool AB>
+template <int R, int NQ, int L, bool AB>
 __global__ __launch_bounds__(64) void k_s2_rows_w2(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb,
     double* __restrict__ Q, int ldq, double* __restrict__ Z, int ldz, int n,
     int s, int nblocks, const double* __restrict__ CU,
     const double* __restrict__ SU, const double* __restrict__ CV,
     const double* __restrict__ SV) {
-  constexpr int NQH = NQ / 2, RP = R + NQH, NW = R + NQ - 1, NP = NW + 2 * NQH, LDT = 33;
-  static_assert(NQ % 2 == 0, "two lanes take half of the sequences each");
-  __shared__ double tile[NP][LDT];  // row NQH + e: window entry e
-  __shared__ TapeBlock<RP, NQH> tb[2];
+  constexpr int NQH = NQ / L, RP = R + (L - 1) * NQH, NW = R + NQ - 1;
+  constexpr int PAD = (L - 1) * NQH, NP = NW + 2 * PAD, CPW = 64 / L, LDT = CPW + 1;
+  static_assert(NQ % L == 0, "the lanes of a row take equal shares of the sequences");
+  __shared__ double tile[NP][LDT];  // row PAD + e: window entry e
+  __shared__ TapeBlock<RP, NQH> tb[L];
   const int which = AB ? blockIdx.y : 2 + blockIdx.y;  // 0 A, 1 B, 2 Q, 3 Z
-  const int row_first = blockIdx.x * 32;
-  const int t = threadIdx.x, c = t >> 1, h = t & 1;
+  const int row_first = blockIdx.x * CPW;
+  const int t = threadIdx.x, c = t / L, h = t % L;
   const int i = row_first + c;  // 0-based row
   const bool valid = i < n;
   double* M = which == 0 ? A : which == 1 ? B : which == 2 ? Q : Z;
@@ -715,9 +714,9 @@ __global__ __launch_bounds__(64) void k_s2_rows_w2(
   const double* Ct = which == 2 ? CU : CV;
   const double* St = which == 2 ? SU : SV;
   const double sgn = which == 2 ? -1.0 : 1.0;
-  for (int e = t; e < NQH * LDT; e += 64) {
+  for (int e = t; e < PAD * LDT; e += 64) {
     tile[e / LDT][e % LDT] = 0.0;
-    tile[NQH + NW + e / LDT][e % LDT] = 0.0;
+    tile[PAD + NW + e / LDT][e % LDT] = 0.0;
   }
   bool carry = false;
   for (int ib = nblocks; ib >= 1; --ib) {
@@ -729,26 +728,26 @@ __global__ __launch_bounds__(64) void k_s2_rows_w2(
     }
     if (AB && row_first + 1 >= imax(s + 1, s + 1 + (ib - 1) * R)) break;
     __syncthreads();  // the previous block's coefficients are consumed
-    stage_tape_split<R, NQ, 64>(tb, Ct, St, ib, sgn);
+    stage_tape_split<R, NQ, L, 64>(tb, Ct, St, ib, sgn);
     double* src = M + PHT_IDX(valid ? i : 0, cw0, ld);
     if (carry) {
 #pragma unroll
-      for (int k = 0; k < NQ - 1; k += 2)
-        if (k + h < NQ - 1) tile[NQH + R + k + h][c] = tile[NQH + k + h][c];
+      for (int k = 0; k < NQ - 1; k += L)
+        if (k + h < NQ - 1) tile[PAD + R + k + h][c] = tile[PAD + k + h][c];
     }
     const int kf = carry ? R : NW;  // fresh entries [0, kf)
 #pragma unroll
-    for (int k0 = 0; k0 < NW; k0 += 32) {
+    for (int k0 = 0; k0 < NW; k0 += 16 * L) {
       double tmp[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int k = k0 + 2 * q + h;
+        const int k = k0 + L * q + h;
         tmp[q] = (valid && k < W && k < kf) ? src[(size_t)k * ld] : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int k = k0 + 2 * q + h;
-        if (k < NW && k < kf) tile[NQH + k][c] = tmp[q];
+        const int k = k0 + L * q + h;
+        if (k < NW && k < kf) tile[PAD + k][c] = tmp[q];
       }
     }
     __syncthreads();
@@ -766,42 +765,43 @@ __global__ __launch_bounds__(64) void k_s2_rows_w2(
     else if (icm0h < NQH)
       wavefront_window<RP, NQH, LDT>(col, tb[h], icm0h);
     if (valid)
-      for (int k = h; k < W; k += 2) src[(size_t)k * ld] = tile[NQH + k][c];
+      for (int k = h; k < W; k += L) src[(size_t)k * ld] = tile[PAD + k][c];
     carry = true;
   }
 }
 
 // Columns of A (which = 0) or B (1) right of the generation window in the
-// split wavefront form, two lanes per column and 32 columns per workgroup
+// split wavefront form, L lanes per column and 64/L columns per workgroup
 // (left rotations in row form: the transposed coefficients).  Global
 // accesses run along the columns through the tile; the tile carries the
 // shared entries between blocks as in k_s2_rows_w2.
-template <int R, int NQ>
+template <int R, int NQ, int L>
 __global__ __launch_bounds__(64) void k_s2_cols_w2(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
     int s, int nblocks, const double* __restrict__ CU,
     const double* __restrict__ SU) {
-  constexpr int NQH = NQ / 2, RP = R + NQH, NW = R + NQ - 1, NP = NW + 2 * NQH, LDT = 33;
-  static_assert(NQ % 2 == 0, "two lanes take half of the sequences each");
+  constexpr int NQH = NQ / L, RP = R + (L - 1) * NQH, NW = R + NQ - 1;
+  constexpr int PAD = (L - 1) * NQH, NP = NW + 2 * PAD, CPW = 64 / L, LDT = CPW + 1;
+  static_assert(NQ % L == 0, "the lanes of a column take equal shares of the sequences");
   static_assert(NW <= 128, "two 64-row chunks per column");
-  __shared__ double tile[NP][LDT];  // row NQH + k: window row k
-  __shared__ TapeBlock<RP, NQH> tb[2];
+  __shared__ double tile[NP][LDT];  // row PAD + k: window row k
+  __shared__ TapeBlock<RP, NQH> tb[L];
   const int which = blockIdx.y;
-  const int t = threadIdx.x, c = t >> 1, h = t & 1;
-  const int j0 = blockIdx.x * 32;
+  const int t = threadIdx.x, c = t / L, h = t % L;
+  const int j0 = blockIdx.x * CPW;
   const int j = j0 + c;
   double* M = which == 0 ? A : B;
   const int ld = which == 0 ? lda : ldb;
-  const int cmax = imin(32, n - j0);
-  for (int e = t; e < NQH * LDT; e += 64) {
+  const int cmax = imin(CPW, n - j0);
+  for (int e = t; e < PAD * LDT; e += 64) {
     tile[e / LDT][e % LDT] = 0.0;
-    tile[NQH + NW + e / LDT][e % LDT] = 0.0;
+    tile[PAD + NW + e / LDT][e % LDT] = 0.0;
   }
   bool carry = false;
   for (int ib = nblocks; ib >= 1; --ib) {
     const int istop = (which == 0) ? ((ib == 1) ? s + NQ - 1 : s + (ib - 2) * R + NQ)
                                    : s + (ib - 1) * R + R + NQ - 1;  // 1-based
-    if (j0 + 32 <= istop) {  // no column of this workgroup takes part
+    if (j0 + CPW <= istop) {  // no column of this workgroup takes part
       carry = false;
       continue;
     }
@@ -812,16 +812,16 @@ __global__ __launch_bounds__(64) void k_s2_cols_w2(
       continue;
     }
     __syncthreads();  // the previous block's coefficients and tile are consumed
-    stage_tape_split<R, NQ, 64>(tb, CU, SU, ib, -1.0);
+    stage_tape_split<R, NQ, L, 64>(tb, CU, SU, ib, -1.0);
     if (carry) {
 #pragma unroll
-      for (int k = 0; k < NQ - 1; k += 2)
-        if (k + h < NQ - 1) tile[NQH + R + k + h][c] = tile[NQH + k + h][c];
+      for (int k = 0; k < NQ - 1; k += L)
+        if (k + h < NQ - 1) tile[PAD + R + k + h][c] = tile[PAD + k + h][c];
     }
     const int kf = carry ? R : NW;  // fresh rows [0, kf)
     // 8 columns x 2 row chunks per group, all 16 loads in flight together
 #pragma unroll
-    for (int c0 = 0; c0 < 32; c0 += 8) {
+    for (int c0 = 0; c0 < CPW; c0 += 8) {
       double tmp[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
@@ -832,7 +832,7 @@ __global__ __launch_bounds__(64) void k_s2_cols_w2(
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int cc = c0 + (q >> 1), k = t + 64 * (q & 1);
-        if (k < NW && k < kf) tile[NQH + k][cc] = tmp[q];
+        if (k < NW && k < kf) tile[PAD + k][cc] = tmp[q];
       }
     }
     __syncthreads();
@@ -841,11 +841,14 @@ __global__ __launch_bounds__(64) void k_s2_cols_w2(
     __syncthreads();
     for (int cc = 0; cc < cmax; ++cc) {
       if (j0 + cc + 1 <= istop) continue;
-      for (int k = t; k < W; k += 64) M[PHT_IDX(rw0 + k, j0 + cc, ld)] = tile[NQH + k][cc];
+      for (int k = t; k < W; k += 64) M[PHT_IDX(rw0 + k, j0 + cc, ld)] = tile[PAD + k][cc];
     }
     carry = true;
   }
 }
+
+// lanes per row (column) in the split wavefront kernels
+constexpr int kLanes = 2;
 
 // windows up to 80 entries in the wavefront kernels
 constexpr bool wavefront_fits(int R, int NQ) { return R + NQ - 1 <= 80; }
@@ -936,13 +939,13 @@ hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
   if (P.r == RR && P.nq == NN) {                                                \
     const dim3 g((P.n + kRowsThreads - 1) / kRowsThreads, 2);                   \
     if (wavefront_fits(RR, NN) && s2_wavefront()) {                             \
-      const dim3 g2((P.n + 31) / 32, 2);                                        \
+      const dim3 g2((P.n + 64 / kLanes - 1) / (64 / kLanes), 2);                \
       if (part != kRowsQZ)                                                      \
-        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, true>), g2, dim3(64), 0, st, A, \
+        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, kLanes, true>), g2, dim3(64), 0, st, A, \
                            lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
                            P.nblocks, CU, SU, CV, SV);                          \
       if (part != kRowsAB)                                                      \
-        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, false>), g2, dim3(64), 0, st, A, \
+        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, kLanes, false>), g2, dim3(64), 0, st, A, \
                            lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
                            P.nblocks, CU, SU, CV, SV);                          \
       return hipGetLastError();                                                 \
@@ -982,7 +985,8 @@ hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
 #define PHT_COLS(RR, NN)                                                     \
   if (P.r == RR && P.nq == NN) {                                             \
     if (wavefront_fits(RR, NN) && s2_wavefront())                            \
-      hipLaunchKernelGGL((k_s2_cols_w2<RR, NN>), dim3((P.n + 31) / 32, 2),   \
+      hipLaunchKernelGGL((k_s2_cols_w2<RR, NN, kLanes>),                     \
+                         dim3((P.n + 64 / kLanes - 1) / (64 / kLanes), 2),   \
                          dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,     \
                          P.nblocks, CU, SU);                                 \
     else                                                                     \
