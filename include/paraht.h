@@ -41,7 +41,10 @@ typedef struct paraht_ctx paraht_ctx;
 int paraht_ctx_create(paraht_ctx** ctx, int device);
 int paraht_ctx_destroy(paraht_ctx* ctx);
 /* Use an existing hipStream_t (as void*) for all work of this context;
- * NULL restores the context's own stream. */
+ * NULL restores the context's own stream.  The library also runs the
+ * updates of Q and Z (and, in stage 1, the left update of A) on a second,
+ * non-blocking stream of its own, ordered against this one by events; every
+ * call makes this stream wait for that work before it returns. */
 int paraht_ctx_set_stream(paraht_ctx* ctx, void* stream);
 /* Phase times (milliseconds) of the last call on this context:
  * [0] stage 1 total, [1] stage-1 left generation, [2] stage-1 right
