@@ -43,10 +43,12 @@ hipError_t s1_rq_gen(hipStream_t st, const double* B, int ldb, int i0,
                      int len, int nb, double* V, int ldv, double* T, int ldt,
                      double* scratch);
 // Register-resident versions for P = p*nb <= 128 and nb <= 64 (rq: nb < len).
+// With tau_ws (nblocks * 64 doubles), the panel's chain of block QRs leaves
+// the T matrices to a second launch that builds them all at once.
 hipError_t s1_panel_qr_fast(hipStream_t st, double* A, int lda, int n, int jb,
                             int nb, int p, int nblocks, double* Vt, int ldv,
                             long long vstride, double* Tt, int ldt,
-                            long long tstride);
+                            long long tstride, double* tau_ws = nullptr);
 hipError_t s1_rq_fast(hipStream_t st, const double* B, int ldb, int i0, int len,
                       int nb, double* V, int ldv, double* T, int ldt);
 // Sets B(j+1:i0+len-1, j) = 0 for j in [i0, i0+ncols).

@@ -320,7 +320,7 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   // the panel tapes are double-buffered: panel k's Q and Z updates run on the
   // second stream while panel k+1 is generated into the other buffers
   const size_t vsz = (size_t)nbmax * P * nb, tsz = (size_t)nbmax * nb * nb;
-  const size_t scr = (size_t)P * P + (size_t)nb * P;
+  const size_t scr = (size_t)P * P + (size_t)nb * P + (size_t)nbmax * 64;  // + panel taus
   const size_t dsz = 5 * nd * sizeof(ApplyBlock);
   size_t bytes = (4 * vsz + 4 * tsz + scr) * sizeof(double) + dsz + 256;
   if (int e = ensure_ws(c, bytes)) return e;
@@ -364,7 +364,8 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     if (fast_gen)
       PROF(PARAHT_K_S1_PANEL_QR, qf, 0.0,
            pht::s1_panel_qr_fast(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
-                                 (long long)P * nb, TL, nb, (long long)nb * nb));
+                                 (long long)P * nb, TL, nb, (long long)nb * nb,
+                                 scratch + (size_t)P * P + (size_t)nb * P));
     else
       PROF(PARAHT_K_S1_PANEL_QR, qf, 0.0,
            pht::s1_panel_qr(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,

@@ -230,10 +230,13 @@ __device__ inline void gram_and_t(const double* Vs, int k, const double* tau, do
 // ---------------------------------------------------------------------------
 // Left generation of one panel (ab2rht.f90:93-118), blocks bottom-up.
 // Thread t owns column t/16 of the block, rows 16*j + t%16 (j < 8).
+// With tau_out, each block's tau goes there (64 per block) and T is left to
+// panel_t_body, one workgroup per block after the chain; otherwise T is
+// built here, block by block.
 __device__ inline void panel_qr_body(
     double* sm, double* __restrict__ A, int lda, int n, int jb, int nb, int p, int nblocks,
     double* __restrict__ Vt, int ldv, long long vstride, double* __restrict__ Tt,
-    int ldt, long long tstride) {
+    int ldt, long long tstride, double* __restrict__ tau_out = nullptr) {
   constexpr int NJ = 8;
   double* Vc = sm;               // [64][kLd]: reflector i in row i
   double* G = Vc + 64 * kLd;     // [64][65]
@@ -325,12 +328,44 @@ __device__ inline void panel_qr_body(
       const int r = e % len, c = e / len;
       Vb[PHT_IDX(r, c, ldv)] = (c < k) ? Vc[c * kLd + r] : 0.0;
     }
-    gram_and_t(Vc, k, tau, G, Tl, Tt + (long long)bl * tstride, ldt, nb);
-    // the next block (above) re-reads the R rows just written to A, and
-    // overwrites Vc / tau
-    __threadfence();
-    __syncthreads();
+    if (tau_out != nullptr) {
+      if (tid < k) tau_out[(size_t)bl * 64 + tid] = tau[tid];
+      // the next block (above) re-reads the R rows just written to A by
+      // this workgroup (the barrier orders them), and overwrites Vc / tau
+      __syncthreads();
+    } else {
+      gram_and_t(Vc, k, tau, G, Tl, Tt + (long long)bl * tstride, ldt, nb);
+      __threadfence();
+      __syncthreads();
+    }
   }
+}
+
+// T of block bl = blockIdx.x of a panel from its stored V and tau (the
+// deferred part of panel_qr_body): V back into zero-padded LDS rows, then
+// gram_and_t.  The blocks are independent, so a panel's T matrices are
+// built in one launch of nblocks workgroups.
+__device__ inline void panel_t_body(double* sm, int n, int jb, int nb, int p, int nblocks,
+                                    const double* __restrict__ Vt, int ldv, long long vstride,
+                                    const double* __restrict__ tau_in,
+                                    double* __restrict__ Tt, int ldt, long long tstride) {
+  double* Vc = sm;               // [64][kLd]
+  double* G = Vc + 64 * kLd;     // [64][65]
+  double* Tl = G + 64 * 65;      // [64][65]
+  double* tau = Tl + 64 * 65;    // [64]
+  const int tid = threadIdx.x, bl = blockIdx.x;
+  const int P = p * nb;
+  const int i0 = jb + nb + (nblocks - 1 - bl) * (p - 1) * nb;
+  const int len = min(n - i0, P);
+  const int k = min(nb, len);
+  const double* Vb = Vt + (long long)bl * vstride;
+  for (int e = tid; e < 64 * 128; e += kT) {
+    const int c = e >> 7, r = e & 127;
+    Vc[c * kLd + r] = (c < k && r < len) ? Vb[PHT_IDX(r, c, ldv)] : 0.0;
+  }
+  if (tid < 64) tau[tid] = (tid < k) ? tau_in[(size_t)bl * 64 + tid] : 0.0;
+  __syncthreads();
+  gram_and_t(Vc, k, tau, G, Tl, Tt + (long long)bl * tstride, ldt, nb);
 }
 
 // ---------------------------------------------------------------------------
