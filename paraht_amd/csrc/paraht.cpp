@@ -447,7 +447,7 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
 int s2_helpers() {
   static int h = [] {
     const char* e = getenv("PARAHT_S2_HELPERS");
-    int v = e ? atoi(e) : 2;
+    int v = e ? atoi(e) : 3;
     return v < 0 ? 0 : (v > pht::kS2MaxHelpers ? pht::kS2MaxHelpers : v);
   }();
   return h;
