@@ -42,7 +42,7 @@ hipError_t s1_panel_qr(hipStream_t st, double* A, int lda, int n, int jb,
 hipError_t s1_rq_gen(hipStream_t st, const double* B, int ldb, int i0,
                      int len, int nb, double* V, int ldv, double* T, int ldt,
                      double* scratch);
-// Register-resident versions for P = p*nb <= 128 and nb <= 64 (rq: nb < len).
+// Register-resident versions for P = p*nb <= 128 and nb <= 64 (rq: any len; len <= nb is the reference's 'B','R' tail block).
 // With tau_ws (nblocks * 64 doubles), the panel's chain of block QRs leaves
 // the T matrices to a second launch that builds them all at once.
 hipError_t s1_panel_qr_fast(hipStream_t st, double* A, int lda, int n, int jb,

@@ -386,7 +386,7 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     for (int bl = 0; bl < pi.nblocks; ++bl) {
       const ApplyBlock& b = BR[pi.first + bl];
       const double rf = b.len > nb ? rq_flops(b.len, nb) : (4.0 / 3.0) * b.len * b.len * b.len;
-      if (fast_gen && b.len > nb)
+      if (fast_gen)
         PROF(PARAHT_K_S1_RQ_GEN, rf, 0.0,
              pht::s1_rq_fast(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
                              TR + (size_t)bl * nb * nb, nb));

@@ -71,7 +71,7 @@ hipError_t s1_panel_qr_fast(hipStream_t st, double* A, int lda, int n, int jb,
 
 hipError_t s1_rq_fast(hipStream_t st, const double* B, int ldb, int i0, int len,
                       int nb, double* V, int ldv, double* T, int ldt) {
-  if (nb > 64 || len > 128 || len <= nb) return hipErrorInvalidValue;
+  if (nb > 64 || len > 128 || len < 1) return hipErrorInvalidValue;
   static bool attr = false;
   if (hipError_t e = allow_lds((const void*)k_s1_rq_fast, kRqLds, attr)) return e;
   hipLaunchKernelGGL(k_s1_rq_fast, dim3(1), dim3(kT), kRqLds, st, B, ldb, i0, len, nb,
