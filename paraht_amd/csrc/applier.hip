@@ -26,13 +26,17 @@ template <int SIDE, int NKT>
 __global__ __launch_bounds__(256) void k_apply_blocks(
     double* __restrict__ C, int ldc, const double* __restrict__ V, int ldv,
     const double* __restrict__ T, int ldt, const ApplyBlock* __restrict__ blocks,
-    int nblocks, int other_begin) {
+    int nblocks, int other_begin, int other_end) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int o0 = other_begin + (blockIdx.x * 4 + wave) * 16;
 
   for (int b = 0; b < nblocks; ++b) {
-    const ApplyBlock bk = blocks[b];
+    ApplyBlock bk = blocks[b];
+    // the launch's range [other_begin, other_end) bounds every block's (the
+    // last workgroup's rows past other_end belong to another launch)
+    bk.lo = bk.lo > other_begin ? bk.lo : other_begin;
+    bk.hi = bk.hi < other_end ? bk.hi : other_end;
     if ((o0 < bk.hi) && (o0 + 16 > bk.lo))
       apply_block_slice<SIDE, NKT>(C, ldc, V, ldv, T, ldt, bk, o0, lane);
     // consecutive blocks overlap in the span dimension: make this wave's
@@ -56,7 +60,7 @@ template <int SIDE, int NKT>
 __global__ __launch_bounds__(256) void k_apply_blocks_lds(
     double* __restrict__ C, int ldc, const double* __restrict__ V, int ldv,
     const double* __restrict__ T, int ldt, const ApplyBlock* __restrict__ blocks,
-    int nblocks, int other_begin) {
+    int nblocks, int other_begin, int other_end) {
   __shared__ double Vs[kStageK * kLdsV];  // V(s, m) at m * kLdsV + s
   __shared__ double Ts[kStageK * kLdsT];  // T(kk, m) at m * kLdsT + kk
   const int tid = threadIdx.x;
@@ -68,7 +72,11 @@ __global__ __launch_bounds__(256) void k_apply_blocks_lds(
   const int oi = o0 + l16;
 
   for (int b = 0; b < nblocks; ++b) {
-    const ApplyBlock bk = blocks[b];
+    ApplyBlock bk = blocks[b];
+    // the launch's range [other_begin, other_end) bounds every block's (the
+    // last workgroup's rows past other_end belong to another launch)
+    bk.lo = bk.lo > other_begin ? bk.lo : other_begin;
+    bk.hi = bk.hi < other_end ? bk.hi : other_end;
     const int len = bk.len, k = bk.k, st = bk.start;
     const bool active = (o0 < bk.hi) && (o0 + 16 > bk.lo);  // uniform per wave
     const bool omask = (oi >= bk.lo) && (oi < bk.hi);
@@ -175,7 +183,7 @@ static hipError_t launch_side(hipStream_t st, double* C, int ldc,
 #define PHT_CASE(N)                                                              \
   case N:                                                                        \
     hipLaunchKernelGGL((k_apply_blocks_lds<SIDE, N>), grid, block, 0, st, C, ldc, \
-                       V, ldv, T, ldt, dblocks, nblocks, other_begin);            \
+                       V, ldv, T, ldt, dblocks, nblocks, other_begin, other_end); \
     return hipGetLastError();
       PHT_CASE(1)
       PHT_CASE(2)
@@ -190,7 +198,7 @@ static hipError_t launch_side(hipStream_t st, double* C, int ldc,
 #define PHT_CASE(N)                                                         \
   case N:                                                                   \
     hipLaunchKernelGGL((k_apply_blocks<SIDE, N>), grid, block, 0, st, C, ldc, \
-                       V, ldv, T, ldt, dblocks, nblocks, other_begin);       \
+                       V, ldv, T, ldt, dblocks, nblocks, other_begin, other_end); \
     break;
     PHT_CASE(1)
     PHT_CASE(2)

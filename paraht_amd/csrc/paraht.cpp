@@ -401,15 +401,25 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
            pht::zero_below(st, B, ldb, b.start, b.len, b.len > nb ? nb : b.len));
     }
     const int m3 = tm.mark();
-    CK(hipStreamWaitEvent(st, c->ev_aleft, 0));
-    PROF(PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
-         pht::apply_blocks(st, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
-                           pi.nblocks, nb, P, 0, n));
-    const int m4 = tm.mark();
-    // Q and Z are written by these updates only: they leave the chain of
-    // panel generations for the second stream (in panel order there)
+    // this panel's right tapes are complete: the second stream takes the rows
+    // of A above the next panel's first row (jb0 + nb), which no later panel
+    // generation reads and no later main-stream update touches before the
+    // next panel's tapes exist, then Q and Z (written by these updates only)
     CK(hipEventRecord(c->ev_tape, st));
     CK(hipStreamWaitEvent(c->aux, c->ev_tape, 0));
+    const int split = std::min(n, pi.jb0 + nb);
+    const double fa = wy_flops(RR, pi.first, pi.nblocks);
+    if (split > 0)
+      PROF_ON(c->aux, PARAHT_K_WY_APPLY, fa * split / n, 0.0,
+              pht::apply_blocks(c->aux, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
+                                pi.nblocks, nb, P, 0, split));
+    // the rows the next panel's QR reads stay on the chain, after this
+    // panel's left update of A (second stream)
+    CK(hipStreamWaitEvent(st, c->ev_aleft, 0));
+    PROF(PARAHT_K_WY_APPLY, fa * (n - split) / n, 0.0,
+         pht::apply_blocks(st, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
+                           pi.nblocks, nb, P, split, n));
+    const int m4 = tm.mark();
     PROF_ON(c->aux, PARAHT_K_WY_APPLY, wy_flops(QL, pi.first, pi.nblocks), 0.0,
             pht::apply_blocks(c->aux, pht::kRightN, Q, ldq, VL, P, TL, nb, dQL + pi.first,
                               pi.nblocks, nb, P, 0, n));
