@@ -56,9 +56,10 @@ SIGNATURES = {
 
 
 def build(verbose: bool = False) -> str:
-    """Compile libparaht_hip.so for gfx950 with hipcc (always recompiles
-    what changed; the Makefile tracks sources and headers)."""
-    cmd = ["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")]
+    """Compile libparaht_hip.so for gfx950 with hipcc: every source, every
+    time (make -B), so a call checks that the tree builds, not only that a
+    library exists."""
+    cmd = ["make", "-s", "-B", "-j8", "-C", os.path.join(_HERE, "csrc")]
     subprocess.run(cmd, check=True,
                    stdout=None if verbose else subprocess.DEVNULL)
     return LIB_PATH
