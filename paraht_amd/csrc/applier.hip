@@ -60,7 +60,7 @@ template <int SIDE, int NKT>
 __global__ __launch_bounds__(256) void k_apply_blocks_lds(
     double* __restrict__ C, int ldc, const double* __restrict__ V, int ldv,
     const double* __restrict__ T, int ldt, const ApplyBlock* __restrict__ blocks,
-    int nblocks, int other_begin, int other_end) {
+    int nblocks, int other_begin, int other_end, int zcols) {
   __shared__ double Vs[kStageK * kLdsV];  // V(s, m) at m * kLdsV + s
   __shared__ double Ts[kStageK * kLdsT];  // T(kk, m) at m * kLdsT + kk
   const int tid = threadIdx.x;
@@ -158,7 +158,10 @@ __global__ __launch_bounds__(256) void k_apply_blocks_lds(
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int sidx = 16 * ct + 4 * i + q;
-            if (omask && sidx < len) C[cidx<SIDE>(st + sidx, oi, ldc)] = acc[i];
+            // zcols (right side): the entries below the diagonal of the
+            // block's first zcols columns are stored as exact zeros
+            const bool z = SIDE == kRightN && sidx < zcols && oi > st + sidx && oi < st + len;
+            if (omask && sidx < len) C[cidx<SIDE>(st + sidx, oi, ldc)] = z ? 0.0 : acc[i];
           }
         }
       }
@@ -173,7 +176,8 @@ template <int SIDE>
 static hipError_t launch_side(hipStream_t st, double* C, int ldc,
                               const double* V, int ldv, const double* T,
                               int ldt, const ApplyBlock* dblocks, int nblocks,
-                              int kmax, int max_len, int other_begin, int other_end) {
+                              int kmax, int max_len, int other_begin, int other_end,
+                              int zcols) {
   const int extent = other_end - other_begin;
   if (extent <= 0 || nblocks <= 0) return hipSuccess;
   dim3 grid((extent + 63) / 64), block(256);
@@ -183,7 +187,7 @@ static hipError_t launch_side(hipStream_t st, double* C, int ldc,
 #define PHT_CASE(N)                                                              \
   case N:                                                                        \
     hipLaunchKernelGGL((k_apply_blocks_lds<SIDE, N>), grid, block, 0, st, C, ldc, \
-                       V, ldv, T, ldt, dblocks, nblocks, other_begin, other_end); \
+                       V, ldv, T, ldt, dblocks, nblocks, other_begin, other_end, zcols); \
     return hipGetLastError();
       PHT_CASE(1)
       PHT_CASE(2)
@@ -215,15 +219,21 @@ static hipError_t launch_side(hipStream_t st, double* C, int ldc,
   return hipGetLastError();
 }
 
+bool apply_fuses_zero(int kmax, int max_len) {
+  return max_len <= kStageLen && kmax <= kStageK && kmax >= 1;
+}
+
 hipError_t apply_blocks(hipStream_t st, Side side, double* C, int ldc,
                         const double* V, int ldv, const double* T, int ldt,
                         const ApplyBlock* dblocks, int nblocks, int kmax, int max_len,
-                        int other_begin, int other_end) {
+                        int other_begin, int other_end, int zero_cols) {
+  if (zero_cols > 0 && (side != kRightN || !apply_fuses_zero(kmax, max_len)))
+    return hipErrorInvalidValue;
   if (side == kLeftT)
     return launch_side<kLeftT>(st, C, ldc, V, ldv, T, ldt, dblocks, nblocks,
-                               kmax, max_len, other_begin, other_end);
+                               kmax, max_len, other_begin, other_end, 0);
   return launch_side<kRightN>(st, C, ldc, V, ldv, T, ldt, dblocks, nblocks,
-                              kmax, max_len, other_begin, other_end);
+                              kmax, max_len, other_begin, other_end, zero_cols);
 }
 
 }  // namespace pht

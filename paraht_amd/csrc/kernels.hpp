@@ -23,10 +23,14 @@ enum Side { kLeftT = 0, kRightN = 1 };
 // (kRightN), on FP64 MFMA tiles.  [other_begin, other_end) bounds the other
 // dimension over all blocks (it sets the grid).  kmax = max k and max_len =
 // max len over the blocks (len <= 128 and k <= 64 take the LDS-staged kernel).
+// zero_cols > 0 (right side, one block, and only where apply_fuses_zero
+// holds): the block's entries below the diagonal in its first zero_cols
+// columns are stored as exact zeros, as zero_below would set them.
 hipError_t apply_blocks(hipStream_t st, Side side, double* C, int ldc,
                         const double* V, int ldv, const double* T, int ldt,
                         const ApplyBlock* dblocks, int nblocks, int kmax, int max_len,
-                        int other_begin, int other_end);
+                        int other_begin, int other_end, int zero_cols = 0);
+bool apply_fuses_zero(int kmax, int max_len);
 
 // ---- stage 1 (ab2rht_simple) ----
 // Bottom-up block QRs of the panel A(jb+nb:n, jb:jb+nb-1) (0-based jb);

@@ -394,11 +394,16 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
         PROF(PARAHT_K_S1_RQ_GEN, rf, 0.0,
              pht::s1_rq_gen(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
                             TR + (size_t)bl * nb * nb, nb, scratch));
+      // the subdiagonal part of the block's first columns becomes exact
+      // zeros inside the update where the applier can (ab2rht.f90:329-333)
+      const int zc = b.len > nb ? nb : b.len;
+      const bool fuse = pht::apply_fuses_zero(b.k, P);
       PROF(PARAHT_K_WY_APPLY, wy_flops(BR, pi.first + bl, 1), 0.0,
            pht::apply_blocks(st, pht::kRightN, B, ldb, VR, P, TR, nb,
-                             dBR + pi.first + bl, 1, b.k, P, 0, b.start + b.len));
-      PROF(PARAHT_K_OTHER, 0.0, 0.0,
-           pht::zero_below(st, B, ldb, b.start, b.len, b.len > nb ? nb : b.len));
+                             dBR + pi.first + bl, 1, b.k, P, 0, b.start + b.len,
+                             fuse ? zc : 0));
+      if (!fuse)
+        PROF(PARAHT_K_OTHER, 0.0, 0.0, pht::zero_below(st, B, ldb, b.start, b.len, zc));
     }
     const int m3 = tm.mark();
     // this panel's right tapes are complete: the second stream takes the rows
