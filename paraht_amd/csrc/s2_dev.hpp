@@ -360,25 +360,34 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
     //      wave 0, lane = block row; the pivot values are read from lane ig
     if (wave == 0) right_chain<RMAX>(Bb, nb, lane, rc, rs);
     __syncthreads();
-    for (int e = tid; e < nb * nb; e += NT) {
-      const int k = e % nb, j = e / nb;
-      B[PHT_IDX(R0 + k, R0 + j, ldb)] = Bb[k][j];
-    }
-    if (tid >= 1 && tid < nb) {
-      const size_t ti = tape_idx(ic + tid - 1, ic, ib, ldr, nq);
-      CV[ti] = rc[tid];
-      SV[ti] = rs[tid];
-    }
 
-    // ---- right rotations on the window rows (rht2ht.f90:853-867,937-951):
-    //      A rows istart..row0+r+nb-1 and B rows istart..row0-1; without
-    //      `above` only A rows row0..row0+r+nb-1 (helpers take the rest)
-    {
-      const int aend = imin(n, row0 + r + nb - 1);  // 1-based inclusive
-      const int first = above ? istart : row0;
-      const int nAr = aend - first + 1;
-      const int nBr = above ? row0 - istart : 0;
-      for (int t = tid; t < nAr + nBr; t += NT) {
+    // ---- the B block and the tape go back, and the right rotations are
+    //      applied to the window rows (rht2ht.f90:853-867,937-951): A rows
+    //      istart..row0+r+nb-1 and B rows istart..row0-1; without `above` only
+    //      A rows row0..row0+r+nb-1 (helpers take the rest).  The window rows
+    //      never include the B block's rows, so when they fit in half the
+    //      workgroup the other half stores the block and the tape meanwhile.
+    const int aend = imin(n, row0 + r + nb - 1);  // 1-based inclusive
+    const int first = above ? istart : row0;
+    const int nAr = aend - first + 1;
+    const int nBr = above ? row0 - istart : 0;
+    const bool halves = nAr + nBr <= NT / 2;
+    const int wt = halves ? NT / 2 : NT;      // threads on the window rows
+    const int st0 = halves ? NT / 2 : 0;      // first thread of the stores
+    if (tid >= st0) {
+      for (int e = tid - st0; e < nb * nb; e += NT - st0) {
+        const int k = e % nb, j = e / nb;
+        B[PHT_IDX(R0 + k, R0 + j, ldb)] = Bb[k][j];
+      }
+      const int tt = tid - st0;
+      if (tt >= 1 && tt < nb) {
+        const size_t ti = tape_idx(ic + tt - 1, ic, ib, ldr, nq);
+        CV[ti] = rc[tt];
+        SV[ti] = rs[tt];
+      }
+    }
+    if (tid < wt) {
+      for (int t = tid; t < nAr + nBr; t += wt) {
         const bool inA = t < nAr;
         rotate_row<RMAX>(inA ? A : B, inA ? lda : ldb, first - 1 + (inA ? t : t - nAr), R0,
                          nb, rc, rs);
