@@ -692,15 +692,19 @@ __device__ __forceinline__ void stage_tape_split(
 // are block ib's [0, NQ-1)), which are carried in the tile, so each later
 // block loads R fresh entries.  AB: rows of A (blockIdx.y = 0) and B (1)
 // above the generation window; otherwise all rows of Q (0) and Z (1).
+// threads of a split-form workgroup: two waves share each block's staged
+// coefficients (the staging is per workgroup, the same for every block)
+constexpr int kSplitThreads = 128;
+
 template <int R, int NQ, int L, bool AB>
-__global__ __launch_bounds__(64) void k_s2_rows_w2(
+__global__ __launch_bounds__(kSplitThreads) void k_s2_rows_w2(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb,
     double* __restrict__ Q, int ldq, double* __restrict__ Z, int ldz, int n,
     int s, int nblocks, const double* __restrict__ CU,
     const double* __restrict__ SU, const double* __restrict__ CV,
     const double* __restrict__ SV) {
   constexpr int NQH = NQ / L, RP = R + (L - 1) * NQH, NW = R + NQ - 1;
-  constexpr int PAD = (L - 1) * NQH, NP = NW + 2 * PAD, CPW = 64 / L, LDT = CPW + 1;
+  constexpr int PAD = (L - 1) * NQH, NP = NW + 2 * PAD, CPW = kSplitThreads / L, LDT = CPW + 1;
   static_assert(NQ % L == 0, "the lanes of a row take equal shares of the sequences");
   __shared__ double tile[NP][LDT];  // row PAD + e: window entry e
   __shared__ TapeBlock<RP, NQH> tb[L];
@@ -714,7 +718,7 @@ __global__ __launch_bounds__(64) void k_s2_rows_w2(
   const double* Ct = which == 2 ? CU : CV;
   const double* St = which == 2 ? SU : SV;
   const double sgn = which == 2 ? -1.0 : 1.0;
-  for (int e = t; e < PAD * LDT; e += 64) {
+  for (int e = t; e < PAD * LDT; e += kSplitThreads) {
     tile[e / LDT][e % LDT] = 0.0;
     tile[PAD + NW + e / LDT][e % LDT] = 0.0;
   }
@@ -728,7 +732,7 @@ __global__ __launch_bounds__(64) void k_s2_rows_w2(
     }
     if (AB && row_first + 1 >= imax(s + 1, s + 1 + (ib - 1) * R)) break;
     __syncthreads();  // the previous block's coefficients are consumed
-    stage_tape_split<R, NQ, L, 64>(tb, Ct, St, ib, sgn);
+    stage_tape_split<R, NQ, L, kSplitThreads>(tb, Ct, St, ib, sgn);
     double* src = M + PHT_IDX(valid ? i : 0, cw0, ld);
     if (carry) {
 #pragma unroll
@@ -776,14 +780,14 @@ __global__ __launch_bounds__(64) void k_s2_rows_w2(
 // accesses run along the columns through the tile; the tile carries the
 // shared entries between blocks as in k_s2_rows_w2.
 template <int R, int NQ, int L>
-__global__ __launch_bounds__(64) void k_s2_cols_w2(
+__global__ __launch_bounds__(kSplitThreads) void k_s2_cols_w2(
     double* __restrict__ A, int lda, double* __restrict__ B, int ldb, int n,
     int s, int nblocks, const double* __restrict__ CU,
     const double* __restrict__ SU) {
   constexpr int NQH = NQ / L, RP = R + (L - 1) * NQH, NW = R + NQ - 1;
-  constexpr int PAD = (L - 1) * NQH, NP = NW + 2 * PAD, CPW = 64 / L, LDT = CPW + 1;
+  constexpr int PAD = (L - 1) * NQH, NP = NW + 2 * PAD, CPW = kSplitThreads / L, LDT = CPW + 1;
   static_assert(NQ % L == 0, "the lanes of a column take equal shares of the sequences");
-  static_assert(NW <= 128, "two 64-row chunks per column");
+  static_assert(NW <= kSplitThreads, "one thread per window row");
   __shared__ double tile[NP][LDT];  // row PAD + k: window row k
   __shared__ TapeBlock<RP, NQH> tb[L];
   const int which = blockIdx.y;
@@ -793,7 +797,7 @@ __global__ __launch_bounds__(64) void k_s2_cols_w2(
   double* M = which == 0 ? A : B;
   const int ld = which == 0 ? lda : ldb;
   const int cmax = imin(CPW, n - j0);
-  for (int e = t; e < PAD * LDT; e += 64) {
+  for (int e = t; e < PAD * LDT; e += kSplitThreads) {
     tile[e / LDT][e % LDT] = 0.0;
     tile[PAD + NW + e / LDT][e % LDT] = 0.0;
   }
@@ -812,26 +816,27 @@ __global__ __launch_bounds__(64) void k_s2_cols_w2(
       continue;
     }
     __syncthreads();  // the previous block's coefficients and tile are consumed
-    stage_tape_split<R, NQ, L, 64>(tb, CU, SU, ib, -1.0);
+    stage_tape_split<R, NQ, L, kSplitThreads>(tb, CU, SU, ib, -1.0);
     if (carry) {
 #pragma unroll
       for (int k = 0; k < NQ - 1; k += L)
         if (k + h < NQ - 1) tile[PAD + R + k + h][c] = tile[PAD + k + h][c];
     }
     const int kf = carry ? R : NW;  // fresh rows [0, kf)
-    // 8 columns x 2 row chunks per group, all 16 loads in flight together
+    // row t of 16 columns per group, all 16 loads in flight together
 #pragma unroll
-    for (int c0 = 0; c0 < CPW; c0 += 8) {
+    for (int c0 = 0; c0 < CPW; c0 += 16) {
       double tmp[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int cc = c0 + (q >> 1), k = t + 64 * (q & 1);
+        const int cc = c0 + q, k = t;
         tmp[q] = (cc < cmax && k < W && k < kf) ? M[PHT_IDX(rw0 + k, j0 + cc, ld)] : 0.0;
       }
-      // the carry copies above precede these writes in the wave's order
+      // every column's carry copy is complete before any fresh row lands
+      if (c0 == 0) __syncthreads();
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int cc = c0 + (q >> 1), k = t + 64 * (q & 1);
+        const int cc = c0 + q, k = t;
         if (k < NW && k < kf) tile[PAD + k][cc] = tmp[q];
       }
     }
@@ -841,7 +846,7 @@ __global__ __launch_bounds__(64) void k_s2_cols_w2(
     __syncthreads();
     for (int cc = 0; cc < cmax; ++cc) {
       if (j0 + cc + 1 <= istop) continue;
-      for (int k = t; k < W; k += 64) M[PHT_IDX(rw0 + k, j0 + cc, ld)] = tile[PAD + k][cc];
+      for (int k = t; k < W; k += kSplitThreads) M[PHT_IDX(rw0 + k, j0 + cc, ld)] = tile[PAD + k][cc];
     }
     carry = true;
   }
@@ -939,13 +944,14 @@ hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
   if (P.r == RR && P.nq == NN) {                                                \
     const dim3 g((P.n + kRowsThreads - 1) / kRowsThreads, 2);                   \
     if (wavefront_fits(RR, NN) && s2_wavefront()) {                             \
-      const dim3 g2((P.n + 64 / kLanes - 1) / (64 / kLanes), 2);                \
+      constexpr int kRows = kSplitThreads / kLanes;                             \
+      const dim3 g2((P.n + kRows - 1) / kRows, 2);                              \
       if (part != kRowsQZ)                                                      \
-        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, kLanes, true>), g2, dim3(64), 0, st, A, \
+        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, kLanes, true>), g2, dim3(kSplitThreads), 0, st, A, \
                            lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
                            P.nblocks, CU, SU, CV, SV);                          \
       if (part != kRowsAB)                                                      \
-        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, kLanes, false>), g2, dim3(64), 0, st, A, \
+        hipLaunchKernelGGL((k_s2_rows_w2<RR, NN, kLanes, false>), g2, dim3(kSplitThreads), 0, st, A, \
                            lda, B, ldb, Q, ldq, Z, ldz, P.n, P.jblock,          \
                            P.nblocks, CU, SU, CV, SV);                          \
       return hipGetLastError();                                                 \
@@ -986,8 +992,8 @@ hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
   if (P.r == RR && P.nq == NN) {                                             \
     if (wavefront_fits(RR, NN) && s2_wavefront())                            \
       hipLaunchKernelGGL((k_s2_cols_w2<RR, NN, kLanes>),                     \
-                         dim3((P.n + 64 / kLanes - 1) / (64 / kLanes), 2),   \
-                         dim3(64), 0, st, A, lda, B, ldb, P.n, P.jblock,     \
+                         dim3((P.n + kSplitThreads / kLanes - 1) / (kSplitThreads / kLanes), 2), \
+                         dim3(kSplitThreads), 0, st, A, lda, B, ldb, P.n, P.jblock, \
                          P.nblocks, CU, SU);                                 \
     else                                                                     \
       hipLaunchKernelGGL((k_s2_cols_v<RR, NN>), dim3((P.n + 63) / 64, 2),    \
