@@ -25,19 +25,29 @@ METRIC = "n=8000 FP64 HT-reduce s & %FP64 peak; batched pencils/s; fwd+bwd/fwd r
 FP64_PEAK = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
 HBM_PEAK = 8.0e12    # bytes/s
 
-# kernel classes timed by the library -> (kernel, roofline bound).  The FP64
-# vector and FP64 matrix peaks of MI355X are the same 78.6 TFLOP/s, so the
-# generators (vector FP64) and the compact-WY applier (FP64 MFMA) share one
-# compute roofline.
+# kernel classes timed by the library -> (kernel, bound).  The throughput
+# kernels are bound by FP64 MFMA (the compact-WY applier) or by HBM (the
+# stage-2 far updates); the generators are chains of dependent steps, bound
+# by latency, and are reported per step instead of against a roofline.
 KERNEL_CLASSES = {
-    "wy_apply": ("k_apply_blocks (compact-WY block reflectors, FP64 MFMA)", "mfma"),
-    "s1_panel_qr": ("k_s1_panel_qr_fast (panel QR + T)", "mfma"),
-    "s1_rq_gen": ("k_s1_rq_fast (RQ + LQ right generator + T)", "mfma"),
-    "s2_gen": ("k_s2_sweep (Givens bulge chase of one sweep)", "mfma"),
-    "s2_rows": ("k_s2_rows_v (rotations on rows of Q, Z and of A, B above the window)", "hbm"),
-    "s2_cols": ("k_s2_cols_v (rotations on columns of A, B right of the window)", "hbm"),
+    "wy_apply": ("k_apply_blocks_lds (compact-WY block reflectors, FP64 MFMA)", "mfma"),
+    "s1_panel_qr": ("k_s1_panel_qr_tau + k_s1_panel_t (panel QR chain + T)", "latency"),
+    "s1_rq_gen": ("k_s1_rq_fast (RQ + LQ right generator + T)", "latency"),
+    "s2_gen": ("k_s2_sweep (pipelined Givens bulge chase of one sweep)", "latency"),
+    "s2_rows": ("k_s2_rows_w2 (rotations on rows of Q, Z and of A, B above the window)", "hbm"),
+    "s2_cols": ("k_s2_cols_w2 (rotations on columns of A, B right of the window)", "hbm"),
 }
-COPY_CAL_DOUBLES = 1 << 27  # 1 GiB read + 1 GiB written per calibration launch
+
+
+def chase_steps(n, r, nq):
+    """Dependent steps of the pipelined sweeps of one stage 2: a sweep of nq
+    columns over nblocks bulge blocks takes nblocks + 3 (nq - 1) steps (each
+    column trails the previous one by the two-step lag and its own step)."""
+    steps = 0
+    for s in range(1, n - 1, nq):
+        nbl = (n - s) // r + ((n - s) % r != 0)
+        steps += nbl + 3 * (min(nq, n - 1 - s) - 1)
+    return steps
 
 
 def traffic_per_launch(cls, args):
@@ -207,10 +217,6 @@ def main():
                     help="pencils timed through the single-pencil path for the loop baseline")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward + backward measurement")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="before the warmup, launch a plain elementwise copy of "
-                         "1 GiB three times (the reference kernel for scaling "
-                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the HT reduction of one pencil is "
@@ -241,16 +247,6 @@ def main():
         Z.copy_(eye)
         torch.cuda.synchronize()
         ph.reduce(A, B, r, p, nq, Q, Z, ctx=ctx)
-
-    if args.copy_calibration:
-        src = torch.ones(COPY_CAL_DOUBLES, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.mul(src, 1.0, out=dst)
-        torch.cuda.synchronize()
-        print(f"copy calibration: 3 launches of {COPY_CAL_DOUBLES * 8} bytes read + "
-              f"{COPY_CAL_DOUBLES * 8} bytes written", file=sys.stderr, flush=True)
-        del src, dst
 
     for i in range(args.warmup):
         step()
@@ -314,7 +310,18 @@ def main():
                           for k, v in kstats.items() if v[1]}
         dom = max(kstats, key=lambda k: kstats[k][0])
         out["dominant_kernel"] = dom
-        if dom in KERNEL_CLASSES:
+        if "s2_gen" in kstats and kstats["s2_gen"][1]:
+            ms_g, cnt_g = kstats["s2_gen"][0], kstats["s2_gen"][1]
+            steps = chase_steps(n, r, nq)
+            out["latency_bound"] = {
+                "kernel": KERNEL_CLASSES["s2_gen"][0],
+                "launches": cnt_g // prof_steps, "avg_launch_us": 1e3 * ms_g / cnt_g,
+                "chase_steps": steps, "us_per_step": 1e3 * ms_g / prof_steps / steps}
+        # the roofline entry: the throughput kernel class with the most time
+        thr = [k for k in kstats if KERNEL_CLASSES.get(k, ("", ""))[1] in ("mfma", "hbm")
+               and kstats[k][1]]
+        if thr:
+            dom = max(thr, key=lambda k: kstats[k][0])
             ms_k, cnt, fl, by = kstats[dom]
             name, bound = KERNEL_CLASSES[dom]
             t_launch = ms_k / cnt / 1e3  # average launch duration (HIP events), s

@@ -133,9 +133,22 @@ def _torch_ready(t) -> None:
     torch.cuda.current_stream(t.device).synchronize()
 
 
+def _check_shapes(n, mats, names, what):
+    """Every matrix must hold an n x n block: at least n rows (the leading
+    dimension is checked separately) and n columns, or the kernels would
+    read and write past its storage."""
+    if not isinstance(n, (int, np.integer)) or n < 0:
+        raise ValueError(f"{what}: illegal value of argument 1 (n={n})")
+    for (m, _), nm in zip(mats, names):
+        shape = tuple(m.shape)
+        if len(shape) != 2 or shape[0] < n or shape[1] < n:
+            raise ValueError(f"{what}: {nm} has shape {shape}, smaller than n x n = {n} x {n}")
+
+
 def _dispatch(host_name, dev_name, what, n, mats, ints, flags, ctx=None):
     L = _lib.lib()
     names = ("A", "B", "Q", "Z")
+    _check_shapes(n, mats, names, what)
     if all(_is_torch(m) for m, _ in mats):
         lds = []
         for (m, ld), nm in zip(mats, names):
@@ -152,8 +165,13 @@ def _dispatch(host_name, dev_name, what, n, mats, ints, flags, ctx=None):
         _lib.check(st, what)
         return
     args = [n]
-    for (m, ld), nm in zip(mats, names):
-        args += [_np_ptr(m, nm), ld if ld is not None else m.shape[0]]
+    for pos, ((m, ld), nm) in enumerate(zip(mats, names)):
+        if ld is not None and ld != m.shape[0]:
+            # argument 3, 5, 7, 9 of the reference's list (lda, ldb, ldq, ldz)
+            raise ValueError(f"{what}: illegal value of argument {3 + 2 * pos} "
+                             f"(ld{nm.lower()}={ld}, the array's leading dimension is "
+                             f"{m.shape[0]})")
+        args += [_np_ptr(m, nm), m.shape[0]]
     st = getattr(L, host_name)(*args, *ints, flags) if flags is not None else \
         getattr(L, host_name)(*args, *ints)
     _lib.check(st, what)
@@ -195,9 +213,12 @@ def reduce(A, B, r, p, nq, Q=None, Z=None, time_components=False, ctx=None):
     NumPy inputs are not modified (copies are reduced); torch CUDA inputs
     (column-major) are reduced in place and returned."""
     flags = PARAHT_TIME_COMPONENTS if time_components else 0
+    n = A.shape[0]
+    for M, nm in ((A, "A"), (B, "B"), (Q, "Q"), (Z, "Z")):
+        if M is not None and tuple(M.shape) != (n, n):
+            raise ValueError(f"reduce: {nm} has shape {tuple(M.shape)}, expected ({n}, {n})")
     if _is_torch(A):
         import torch
-        n = A.shape[0]
         if Q is None:
             Q = column_major(torch.eye(n, dtype=torch.float64, device=A.device))
         if Z is None:
@@ -206,7 +227,6 @@ def reduce(A, B, r, p, nq, Q=None, Z=None, time_components=False, ctx=None):
                   [(A, None), (B, None), (Q, None), (Z, None)], [r, p, nq],
                   flags, ctx)
         return A, B, Q, Z
-    n = A.shape[0]
     H = np.array(A, dtype=np.float64, order="F", copy=True)
     T = np.array(B, dtype=np.float64, order="F", copy=True)
     Q = np.eye(n, order="F") if Q is None else np.array(Q, dtype=np.float64, order="F", copy=True)

@@ -55,6 +55,8 @@ hipError_t s1_panel_qr_fast(hipStream_t st, double* A, int lda, int n, int jb,
                             long long tstride, double* tau_ws = nullptr);
 hipError_t s1_rq_fast(hipStream_t st, const double* B, int ldb, int i0, int len,
                       int nb, double* V, int ldv, double* T, int ldt);
+// Whether the stage-1 generators take blocks of p*nb rows and nb reflectors.
+bool s1_supported(int nb, int p);
 // Sets B(j+1:i0+len-1, j) = 0 for j in [i0, i0+ncols).
 hipError_t zero_below(hipStream_t st, double* B, int ldb, int i0, int len,
                       int ncols);
@@ -69,7 +71,10 @@ struct S2Params {
   int* hprog;            // nq x kS2MaxHelpers step counters of the helper workgroups
   int nhelp;             // helper workgroups per column, 0..kS2MaxHelpers
   int* errflag;          // set when a workgroup's wait hit the spin limit
+  int spin;              // polls before a wait gives up (kS2SpinLimit)
+  int clear_err;         // the call's first sweep: s2_prepare clears *errflag
 };
+constexpr int kS2SpinLimit = 1 << 26;
 constexpr int kS2MaxHelpers = 4;
 // ints of the sweep's counters: progress, hprog (contiguous) and errflag
 inline size_t s2_counter_ints(int nq) { return (size_t)nq * (1 + kS2MaxHelpers) + 64; }
@@ -83,18 +88,54 @@ hipError_t s2_prepare(hipStream_t st, const S2Params& P, double* CU, double* SU,
 hipError_t s2_generate(hipStream_t st, double* A, int lda, double* B, int ldb,
                        const S2Params& P, double* CU, double* SU, double* CV,
                        double* SV);
+// Ranges of a sweep's A/B far update.  Rows above the window: block ib's
+// sequence icm (0-based) applies to the rows i (0-based) with i < row_end and
+//   thr(ib - lo_shift, icm) <= i+1 < thr(ib - hi_shift, icm),
+//   thr(x, icm) = max(s+1, s+1+(x-nq+icm) r)   (the window's first row),
+// lo_shift < 0 meaning no lower bound.  Columns right of the window: block ib
+// applies to the columns j with istop(ib) + c_lo r < j+1 <= istop(ib) + c_hi r
+// (c_hi < 0: no upper bound); the grid starts at column col_begin (0-based).
+// The default is the whole far update.
+struct S2Range {
+  int lo_shift = -1, hi_shift = 0, row_end = 1 << 30;
+  int c_lo = 0, c_hi = -1;
+  int col_begin = 0;
+};
 // Remaining right rotations (rows of A,B above the generation window, all
 // rows of Z) and left rotations seen from the right (all rows of Q); part
-// selects the A/B rows, the Q/Z rows or both.
+// selects the A/B rows, the Q/Z rows or both.  The A/B rows can be limited
+// to a range (s2_far_split) and to the grid's rows [0, row_end).
 enum RowsPart { kRowsAll = 0, kRowsAB = 1, kRowsQZ = 2 };
 hipError_t s2_apply_rows(hipStream_t st, double* A, int lda, double* B,
                          int ldb, double* Q, int ldq, double* Z, int ldz,
                          const S2Params& P, const double* CU, const double* SU,
-                         const double* CV, const double* SV, RowsPart part);
-// Remaining left rotations (columns of A,B right of the generation window).
+                         const double* CV, const double* SV, RowsPart part,
+                         const S2Range& rg = S2Range{}, int row_end = -1);
+// Remaining left rotations (columns of A,B right of the generation window),
+// limited to a range (s2_far_split).
 hipError_t s2_apply_cols(hipStream_t st, double* A, int lda, double* B,
                          int ldb, const S2Params& P, const double* CU,
-                         const double* SU);
+                         const double* SU, const S2Range& rg = S2Range{});
+// Whether (r, nq) has far-update kernels that take ranges (the pairs of
+// PHT_S2_PAIRS); the others run the whole far update in one launch.
+bool s2_far_split(int r, int nq);
+
+// (r, nq) pairs with register-window far-update kernels
+#define PHT_S2_PAIRS(X) \
+  X(8, 4) X(8, 8) X(16, 4) X(16, 8) X(16, 16) X(32, 8) X(32, 16) X(32, 32) \
+  X(64, 8) X(64, 16) X(64, 32)
+// Arguments of the far-update launchers for one sweep.
+struct S2FarArgs {
+  double *A, *B, *Q, *Z;
+  int lda, ldb, ldq, ldz;
+  int n, s, nblocks;
+  const double *CU, *SU, *CV, *SV;
+};
+template <int R, int NQ>
+hipError_t s2_far_rows(hipStream_t st, const S2FarArgs& a, bool ab, const S2Range& rg,
+                       int row_end);
+template <int R, int NQ>
+hipError_t s2_far_cols(hipStream_t st, const S2FarArgs& a, const S2Range& rg);
 
 // ---- batched reduction of many small pencils ----
 // One pencil of a batch: its n x n column-major matrices (ld = n) start at

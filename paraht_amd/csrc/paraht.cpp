@@ -37,8 +37,12 @@ struct paraht_ctx {
   // second stream for work off the critical path (stage 2's Q/Z updates),
   // joined back into `stream` before a call returns
   hipStream_t aux = nullptr;
-  hipEvent_t ev_tape = nullptr;   // a sweep's tape is complete (stream -> aux)
-  hipEvent_t ev_free[2] = {nullptr, nullptr};  // aux is done with tape buffer b
+  // third stream: stage 2's Q/Z updates, beside the A/B far updates on aux
+  hipStream_t aux2 = nullptr;
+  hipEvent_t ev_tape = nullptr;   // a sweep's tape is complete (stream -> aux2)
+  hipEvent_t ev_free[2] = {nullptr, nullptr};  // aux2 (aux in stage 1) is done with tape buffer b
+  hipEvent_t ev_near = nullptr;   // a sweep's lookahead updates are done (stream -> aux)
+  hipEvent_t ev_far = nullptr;    // a sweep's remaining A/B updates are done (aux -> stream)
   hipEvent_t ev_left = nullptr;   // a panel's left tapes are complete (stream -> aux)
   hipEvent_t ev_aleft = nullptr;  // aux has applied a panel's left update to A
   void* ws = nullptr;
@@ -110,6 +114,7 @@ int prof_flush(paraht_ctx* c) {
   if (c->recs.empty()) return 0;
   CK(hipStreamSynchronize(c->stream));
   if (c->aux) CK(hipStreamSynchronize(c->aux));
+  if (c->aux2) CK(hipStreamSynchronize(c->aux2));
   for (const KernelRec& r : c->recs) {
     float t = 0.f;
     CK(hipEventElapsedTime(&t, c->pool[r.e0], c->pool[r.e1]));
@@ -183,7 +188,10 @@ int ensure_aux(paraht_ctx* c) {
   CK(hipEventCreateWithFlags(&c->ev_free[1], hipEventDisableTiming));
   CK(hipEventCreateWithFlags(&c->ev_left, hipEventDisableTiming));
   CK(hipEventCreateWithFlags(&c->ev_aleft, hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->ev_near, hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->ev_far, hipEventDisableTiming));
   CK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+  CK(hipStreamCreateWithFlags(&c->aux2, hipStreamNonBlocking));
   return 0;
 }
 
@@ -242,14 +250,35 @@ double s2_gen_flops(int n, int r, int nq, int s) {
 
 // algorithmic bytes of one sweep's column updates: each block's rows of A
 // and B right of the window, read and written once
-double s2_cols_bytes(int n, int r, int nq, int s, int nblocks) {
+double s2_cols_bytes(int n, int r, int nq, int s, int nblocks, const pht::S2Range& rg) {
   double b = 0.0;
+  auto span = [&](int istop) {
+    const int lo = istop + rg.c_lo * r;
+    const int hi = rg.c_hi < 0 ? n : std::min(n, istop + rg.c_hi * r);
+    return std::max(0, hi - std::max(0, lo));
+  };
   for (int ib = 1; ib <= nblocks; ++ib) {
     const int W = std::min(r + nq - 1, n - (s + (ib - 1) * r));
     if (W <= 0) continue;
     const int istopA = (ib == 1) ? s + nq - 1 : s + (ib - 2) * r + nq;
     const int istopB = s + (ib - 1) * r + r + nq - 1;
-    b += 16.0 * W * (std::max(0, n - istopA) + std::max(0, n - istopB));
+    b += 16.0 * W * (span(istopA) + span(istopB));
+  }
+  return b;
+}
+
+// algorithmic bytes of one sweep's A/B row updates in a range: each row's
+// entries of a block's window, read and written once per block that touches
+// it (rows thr(ib - lo_shift, 0) - 1 <= i < thr(ib - hi_shift, nq-1) - 1)
+double s2_rows_ab_bytes(int n, int r, int nq, int s, int nblocks, const pht::S2Range& rg) {
+  double b = 0.0;
+  auto thr = [&](int x, int icm) { return std::max(s + 1, s + 1 + (x - nq + icm) * r); };
+  for (int ib = 1; ib <= nblocks; ++ib) {
+    const int W = std::min(r + nq - 1, n - (s + (ib - 1) * r));
+    if (W <= 0) continue;
+    const int hi = std::min({n, rg.row_end, thr(ib - rg.hi_shift, nq - 1) - 1});
+    const int lo = rg.lo_shift < 0 ? 0 : thr(ib - rg.lo_shift, 0) - 1;
+    b += 32.0 * W * std::max(0, hi - lo);
   }
   return b;
 }
@@ -468,27 +497,80 @@ int s2_helpers() {
   return h;
 }
 
+// Stage-2 lookahead (rht2ht.f90:1630-1772): a sweep's A/B far update is split
+// into a lookahead part, applied on the main stream before the next sweep's
+// generation, and the rest, applied on the second stream beside it (the
+// reference's bulk update under the A(2,2)/B(2,2) tokens,
+// rht2ht.f90:1803-1958).
+//  * Rows above the window: the lookahead takes, for block ib's sequence icm,
+//    the rows within s2_look_rows(nq) blocks of the window's first row
+//    (thr(ib - Lr, icm) <= i+1 < thr(ib, icm)), the rest the rows above.
+//  * Columns right of the window: the lookahead takes the columns within
+//    s2_look_cols(nq) blocks of each block's istop, the rest the columns
+//    beyond.
+// Order: lookahead rows, lookahead columns (main stream), then rest rows,
+// rest columns (second stream).  A right rotation on columns (y, y+1) and a
+// left rotation on rows (x, x+1) that share entries must each be applied
+// whole on one side of the other; with Lr >= Lc + 1 every pair a boundary
+// cuts has its partner on the consistent side, and the split reproduces the
+// one-pass update bit for bit.  nq and nq+1 blocks keep the next
+// generation's entries out of the rest (tests/test_lookahead_split.py checks
+// the index sets and the order).
+int s2_look_cols(int nq) { return nq + 2; }
+int s2_look_rows(int nq) { return s2_look_cols(nq) + 1; }
+int s2_lookahead() {
+  const char* e = getenv("PARAHT_S2_LOOKAHEAD");
+  if (e && *e == '0') return 0;
+  if (e && *e == '2') return 2;
+  return 1;
+}
+// polls before a wait of the call's first sweep gives up
+// (PARAHT_DEBUG_SPIN0; the error path's test sets it tiny)
+int s2_first_spin() {
+  const char* e = getenv("PARAHT_DEBUG_SPIN0");
+  if (e && *e) {
+    const int v = atoi(e);
+    if (v > 0) return v;
+  }
+  return pht::kS2SpinLimit;
+}
+
+// the limits of the stage-2 kernels (checked before any work of a call)
+int s2_check(int r, int nq) {
+  if (r > 64) return PARAHT_ERR_UNSUPPORTED;
+  if (pht::s2_sweep_lds(r, nq) == 0) return PARAHT_ERR_UNSUPPORTED;  // M of a block in LDS
+  return 0;
+}
+
 int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
            double* Q, int ldq, double* Z, int ldz, int r, int nq, int flags) {
   if (n <= 2) return 0;  // rht2ht.f90:1417 loop is empty
-  if (r > 64) return PARAHT_ERR_UNSUPPORTED;
+  if (int e = s2_check(r, nq)) return e;
   hipStream_t st = c->stream;
   const int ldr = r + nq;
-  if (pht::s2_sweep_lds(r, nq) == 0) return PARAHT_ERR_UNSUPPORTED;  // M of a block in LDS
   int nbl = (n - 1) / r + ((n - 1) % r != 0);
   const size_t tsz = (size_t)ldr * nq * nbl;
   const size_t msz = (size_t)ldr * ldr * nbl;
-  // two tape buffers: the Q/Z updates of sweep k run on the second stream
-  // while sweep k+1 is generated into the other buffer
+  // two tape buffers: the Q/Z updates (and the A/B rest) of sweep k run on
+  // the other streams while sweep k+1 is generated into the other buffer
   if (int e = ensure_ws(c, (8 * tsz + msz) * sizeof(double) + pht::s2_counter_ints(nq) * sizeof(int)))
     return e;
   if (int e = ensure_aux(c)) return e;
   double* tape[2] = {(double*)c->ws, (double*)c->ws + 4 * tsz};
   double* Mws = tape[1] + 4 * tsz;
   const bool tc = (flags & PARAHT_TIME_COMPONENTS) != 0;
+  const int look = s2_lookahead();
+  const bool split = pht::s2_far_split(r, nq) && look != 0;
+  hipStream_t rest = look == 2 ? st : c->aux;  // the stream of the A/B rest
+  const int Lc = s2_look_cols(nq), Lr = s2_look_rows(nq);
   Timer tm(c, tc);
   std::vector<int> marks;
   int buf = 0;
+  bool far_pending = false;
+  // aux2 starts after everything enqueued on the call's stream so far
+  // (stage 1's updates of Q and Z included)
+  CK(hipEventRecord(c->ev_tape, st));
+  CK(hipStreamWaitEvent(c->aux2, c->ev_tape, 0));
   for (int s = 1, k = 0; s <= n - 2; s += nq, ++k) {
     buf = k & 1;
     double* CU = tape[buf];
@@ -510,32 +592,74 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     P.nhelp = (nq <= r + 1) ? std::min(s2_helpers(), c->cu_count / nq - 1) : 0;
     if (P.nhelp < 0) P.nhelp = 0;
     P.errflag = P.progress + (size_t)nq * (1 + pht::kS2MaxHelpers);
+    P.spin = (k == 0) ? s2_first_spin() : pht::kS2SpinLimit;
+    P.clear_err = (k == 0);
     const int m0 = tm.mark();
     // the buffer's previous tape (sweep k-2) must be consumed by its Q/Z update
+    // (its A/B rest finished before sweep k-1's lookahead update)
     if (k >= 2) CK(hipStreamWaitEvent(st, c->ev_free[buf], 0));
     PROF(PARAHT_K_OTHER, 0.0, 0.0, pht::s2_prepare(st, P, CU, SU, CV, SV));
     PROF(PARAHT_K_S2_GEN, s2_gen_flops(n, r, nq, s), 0.0,
          pht::s2_generate(st, A, lda, B, ldb, P, CU, SU, CV, SV));
     const int m1 = tm.mark();
-    // algorithmic traffic: every element of the sweep's columns s+1..n of
-    // Q and Z, and of A and B above the window, is read and written once
     const double w = (double)(n - s);
-    const double rows_ab = std::min<double>(n, s + (double)(P.nblocks - 1) * r);
     CK(hipEventRecord(c->ev_tape, st));
-    CK(hipStreamWaitEvent(c->aux, c->ev_tape, 0));
-    PROF_ON(c->aux, PARAHT_K_S2_ROWS, 0.0, 16.0 * w * 2.0 * n,
-            pht::s2_apply_rows(c->aux, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
+    CK(hipStreamWaitEvent(c->aux2, c->ev_tape, 0));
+    // algorithmic traffic: every element of the sweep's columns s+1..n of Q
+    // and Z is read and written once
+    PROF_ON(c->aux2, PARAHT_K_S2_ROWS, 0.0, 16.0 * w * 2.0 * n,
+            pht::s2_apply_rows(c->aux2, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
                                pht::kRowsQZ));
-    CK(hipEventRecord(c->ev_free[buf], c->aux));
-    PROF(PARAHT_K_S2_ROWS, 0.0, 16.0 * w * 2.0 * rows_ab,
-         pht::s2_apply_rows(st, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
-                            pht::kRowsAB));
-    PROF(PARAHT_K_S2_COLS, 0.0, s2_cols_bytes(n, r, nq, s, P.nblocks),
-         pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU));
+    CK(hipEventRecord(c->ev_free[buf], c->aux2));
+    const int rows_all = std::min(n, s + (P.nblocks - 1) * r);  // rows above some window
+    const int col0 = std::min(n, s + nq - 1);                  // first column right of one
+    if (split) {
+      // the previous sweep's rest must be done before this lookahead
+      if (far_pending) CK(hipStreamWaitEvent(st, c->ev_far, 0));
+      pht::S2Range nr, nc;
+      nr.lo_shift = Lr;
+      nr.hi_shift = 0;
+      nr.row_end = rows_all;
+      nc.c_lo = 0;
+      nc.c_hi = Lc;
+      nc.col_begin = col0;
+      PROF(PARAHT_K_S2_ROWS, 0.0, s2_rows_ab_bytes(n, r, nq, s, P.nblocks, nr),
+           pht::s2_apply_rows(st, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
+                              pht::kRowsAB, nr, rows_all));
+      PROF(PARAHT_K_S2_COLS, 0.0, s2_cols_bytes(n, r, nq, s, P.nblocks, nc),
+           pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU, nc));
+      CK(hipEventRecord(c->ev_near, st));
+      CK(hipStreamWaitEvent(rest, c->ev_near, 0));
+      pht::S2Range fr, fc;
+      fc.c_lo = Lc;
+      fc.c_hi = -1;
+      fc.col_begin = std::min(n, col0 + Lc * r);
+      // rows i+1 < thr(nblocks - Lr, nq - 1)
+      const int far_rows = std::min(n, std::max(s, s + (P.nblocks - 1 - Lr) * r));
+      fr.hi_shift = Lr;
+      fr.row_end = far_rows;
+      PROF_ON(rest, PARAHT_K_S2_ROWS, 0.0, s2_rows_ab_bytes(n, r, nq, s, P.nblocks, fr),
+              pht::s2_apply_rows(rest, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
+                                 pht::kRowsAB, fr, far_rows));
+      PROF_ON(rest, PARAHT_K_S2_COLS, 0.0, s2_cols_bytes(n, r, nq, s, P.nblocks, fc),
+              pht::s2_apply_cols(rest, A, lda, B, ldb, P, CU, SU, fc));
+      CK(hipEventRecord(c->ev_far, rest));
+      far_pending = true;
+    } else {
+      pht::S2Range all, allc;
+      allc.col_begin = pht::s2_far_split(r, nq) ? col0 : 0;
+      PROF(PARAHT_K_S2_ROWS, 0.0, s2_rows_ab_bytes(n, r, nq, s, P.nblocks, all),
+           pht::s2_apply_rows(st, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
+                              pht::kRowsAB, all, rows_all));
+      PROF(PARAHT_K_S2_COLS, 0.0, s2_cols_bytes(n, r, nq, s, P.nblocks, allc),
+           pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU, allc));
+    }
     const int m2 = tm.mark();
     marks.insert(marks.end(), {m0, m1, m2});
   }
-  // join: the last Q/Z update (aux runs its updates in order)
+  // join: the last A/B rest and the last Q/Z update (each stream runs its
+  // updates in order)
+  if (far_pending) CK(hipStreamWaitEvent(st, c->ev_far, 0));
   CK(hipStreamWaitEvent(st, c->ev_free[buf], 0));
   {
     int err = 0;
@@ -555,9 +679,11 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
       app += tm.ms(marks[i + 1], marks[i + 2]);
     }
     c->times[4] = gen;
-    // same line layout as rht2ht.f90:2045-2046 (seconds)
+    // same line layout as rht2ht.f90:2045-2046 (seconds): the lookahead is
+    // the main stream's share of the A/B far updates, the application the
+    // rest (measured on the main stream only when the split is off)
     printf("      gen: %16.6f ;lookahead: %16.6f ;application: %16.6f\n",
-           gen * 1e-3, 0.0, app * 1e-3);
+           gen * 1e-3, split ? app * 1e-3 : 0.0, split ? 0.0 : app * 1e-3);
   }
   return 0;
 }
@@ -843,11 +969,15 @@ int paraht_ctx_destroy(paraht_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->own) (void)hipStreamSynchronize(ctx->own);
   if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
+  if (ctx->aux2) (void)hipStreamSynchronize(ctx->aux2);
   if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
   if (ctx->ws) (void)hipFree(ctx->ws);
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->own) (void)hipStreamDestroy(ctx->own);
   if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+  if (ctx->aux2) (void)hipStreamDestroy(ctx->aux2);
+  if (ctx->ev_near) (void)hipEventDestroy(ctx->ev_near);
+  if (ctx->ev_far) (void)hipEventDestroy(ctx->ev_far);
   if (ctx->ev_tape) (void)hipEventDestroy(ctx->ev_tape);
   if (ctx->ev_left) (void)hipEventDestroy(ctx->ev_left);
   if (ctx->ev_aleft) (void)hipEventDestroy(ctx->ev_aleft);
@@ -899,6 +1029,7 @@ int paraht_ab2rht(int n, double* A, int lda, double* B, int ldb, double* Q,
   if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
   if (nb < 1) return -10;
   if (p < 2) return -11;
+  if (!pht::s1_supported(nb, p)) return PARAHT_ERR_UNSUPPORTED;
   return run_host(kStage1, n, A, lda, B, ldb, Q, ldq, Z, ldz, nb, p, 1, flags);
 }
 
@@ -907,6 +1038,7 @@ int paraht_rht2ht(int n, double* A, int lda, double* B, int ldb, double* Q,
   if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
   if (r < 1) return -10;
   if (nq < 1) return -11;
+  if (int e = s2_check(r, nq)) return e;
   return run_host(kStage2, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, 2, nq, flags);
 }
 
@@ -914,6 +1046,7 @@ int paraht_rht2ht_unblocked(int n, double* A, int lda, double* B, int ldb,
                             double* Q, int ldq, double* Z, int ldz, int r) {
   if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
   if (r < 1) return -10;
+  if (int e = s2_check(r, 1)) return e;
   // one column per sweep: every rotation is applied before the next column
   return run_host(kStage2, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, 2, 1, 0);
 }
@@ -924,6 +1057,8 @@ int paraht_reduce(int n, double* A, int lda, double* B, int ldb, double* Q,
   if (r < 1) return -10;
   if (p < 2) return -11;
   if (nq < 1) return -12;
+  if (!pht::s1_supported(r, p)) return PARAHT_ERR_UNSUPPORTED;
+  if (int e = s2_check(r, nq)) return e;  // before stage 1 changes anything
   return run_host(kBoth, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, p, nq, flags);
 }
 
@@ -934,6 +1069,7 @@ int paraht_ab2rht_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
   if (int e = check_common(n, lda, ldb, ldq, ldz)) return e - 1;
   if (nb < 1) return -11;
   if (p < 2) return -12;
+  if (!pht::s1_supported(nb, p)) return PARAHT_ERR_UNSUPPORTED;
   return run_dev(ctx, kStage1, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, nb, p, 1, flags);
 }
 
@@ -944,6 +1080,7 @@ int paraht_rht2ht_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
   if (int e = check_common(n, lda, ldb, ldq, ldz)) return e - 1;
   if (r < 1) return -11;
   if (nq < 1) return -12;
+  if (int e = s2_check(r, nq)) return e;
   return run_dev(ctx, kStage2, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, 2, nq, flags);
 }
 
@@ -955,6 +1092,8 @@ int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
   if (r < 1) return -11;
   if (p < 2) return -12;
   if (nq < 1) return -13;
+  if (!pht::s1_supported(r, p)) return PARAHT_ERR_UNSUPPORTED;
+  if (int e = s2_check(r, nq)) return e;  // before stage 1 changes anything
   return run_dev(ctx, kBoth, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, p, nq, flags);
 }
 

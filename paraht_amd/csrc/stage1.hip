@@ -313,6 +313,13 @@ __global__ void k_zero_below(double* B, int ldb, int i0, int len, int ncols) {
 // ---------------------------------------------------------------------------
 static constexpr size_t kMaxLds = 160 * 1024;
 
+bool s1_supported(int nb, int p) {
+  if (nb <= 64 && (size_t)p * nb <= 128) return true;  // register-resident generators
+  if (nb > 128) return false;
+  // the generic panel QR keeps a P x nb block in LDS
+  return ((size_t)p * nb * nb + 2 * (size_t)nb + 32) * sizeof(double) <= kMaxLds;
+}
+
 hipError_t s1_panel_qr(hipStream_t st, double* A, int lda, int n, int jb,
                        int nb, int p, int nblocks, double* Vt, int ldv,
                        long long vstride, double* Tt, int ldt,

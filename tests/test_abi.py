@@ -47,3 +47,36 @@ def test_argument_errors_are_reported_without_a_device():
         ph.ab2rht_simple(4, A, 4, A, 4, A, 4, A, 4, 2, 1)
     with pytest.raises(ValueError, match="argument 10"):
         ph.rht2ht_givens(4, A, 4, A, 4, A, 4, A, 4, 0, 2)
+
+
+def test_shapes_smaller_than_n_are_rejected_before_any_call():
+    """A matrix with fewer than n rows or columns would let the kernels (and
+    the host copies) run past its storage: the wrapper refuses it."""
+    import numpy as np
+    import paraht_amd as ph
+    import pytest
+    n = 6
+    A = np.zeros((n, n), order="F")
+    thin = np.zeros((n, n - 2), order="F")
+    short = np.zeros((n - 1, n), order="F")
+    with pytest.raises(ValueError, match="smaller than n x n"):
+        ph.ab2rht_simple(n, A, n, thin, n, A, n, A, n, 2, 2)
+    with pytest.raises(ValueError, match="smaller than n x n"):
+        ph.rht2ht_givens(n, A, n, A, n, short, n - 1, A, n, 2, 2)
+    with pytest.raises(ValueError, match="argument 7"):
+        ph.rht2ht_givens(n, A, n, A, n, A, n + 3, A, n, 2, 2)
+    with pytest.raises(ValueError, match="expected"):
+        ph.reduce(A, thin, 2, 2, 2)
+    with pytest.raises(ValueError, match="expected"):
+        ph.reduce(np.zeros((n, n + 1)), A, 2, 2, 2)
+
+
+def test_torch_shapes_smaller_than_n_are_rejected():
+    import pytest
+    import torch
+    import paraht_amd as ph
+    n = 5
+    A = torch.zeros(n, n, dtype=torch.float64)
+    with pytest.raises(ValueError, match="smaller than n x n"):
+        ph._check_shapes(n, [(A, None), (torch.zeros(n, n - 1), None)], ("A", "B"), "x")
+    ph._check_shapes(n, [(A, None), (torch.zeros(n + 2, n), None)], ("A", "B"), "x")

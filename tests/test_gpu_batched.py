@@ -50,11 +50,15 @@ def test_batched_matches_oracle(oracle_mod):
 
 
 def test_batched_uneven_sizes_invariants():
+    """The full batched workload: 2048 pencils with n uniform in [32, 512],
+    every pencil checked.  With more pencils than compute units the work
+    queue hands each workgroup several pencils."""
     import torch
     from paraht_amd.batched import Batch, reduce_batch
     rng = np.random.default_rng(1302)
-    sizes = [int(x) for x in rng.integers(32, 513, size=192)]
-    print("batched invariants, 192 pencils", flush=True)
+    sizes = [int(x) for x in rng.integers(32, 513, size=2048)]
+    assert len(sizes) > torch.cuda.get_device_properties(0).multi_processor_count
+    print("batched invariants, 2048 pencils", flush=True)
     g = torch.Generator(device="cuda:0")
     g.manual_seed(1302)
     As, Bs = [], []
