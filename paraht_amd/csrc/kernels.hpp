@@ -33,6 +33,35 @@ hipError_t apply_blocks(hipStream_t st, Side side, double* C, int ldc,
 bool apply_fuses_zero(int kmax, int max_len);
 
 // ---- stage 1 (ab2rht_simple) ----
+// The panel at column jb0 (0-based; jb = jb0+1 of ab2rht.f90:71) has
+// nblocks blocks of p*nb rows overlapping by nb rows; block bl (0-based) is
+// at row i0 = jb0 + nb + (nblocks-1-bl) (p-1) nb, bottom-up
+// (ab2rht.f90:74-75,94-99).
+__host__ __device__ inline int s1_nblocks(int n, int nb, int p, int jb0) {
+  const int m = n - nb - jb0, d = (p - 1) * nb;
+  return m / d + (m % d != 0);
+}
+__host__ __device__ inline int s1_npanels(int n, int nb) {
+  return n - nb - 2 >= 0 ? (n - nb - 2) / nb + 1 : 0;
+}
+struct S1Block {
+  int i0, len, k, kr;  // first row, rows, left reflectors, right reflectors
+};
+__host__ __device__ inline S1Block s1_block(int n, int nb, int p, int jb0, int nblocks, int bl) {
+  S1Block b;
+  b.i0 = jb0 + nb + (nblocks - 1 - bl) * (p - 1) * nb;
+  b.len = n - b.i0 < p * nb ? n - b.i0 : p * nb;
+  b.k = nb < b.len ? nb : b.len;
+  b.kr = b.len > nb ? nb : b.len;
+  return b;
+}
+// Writes the compact-WY block lists of every panel into five consecutive
+// arrays of nd blocks (nd = the sum of the panels' nblocks): left updates of
+// B (columns i0..n), of A (columns jb0+nb..n), of Q (rows 0..n), right
+// updates of A and Z (rows 0..n) and of B (rows 0..i0+len).  V and T of
+// block bl sit at bl*p*nb*nb and bl*nb*nb of the panel's tapes.
+hipError_t s1_block_lists(hipStream_t st, ApplyBlock* lists, int nd, int n, int nb, int p);
+
 // Bottom-up block QRs of the panel A(jb+nb:n, jb:jb+nb-1) (0-based jb);
 // writes explicit V (ldv = Pmax rows per block, vstride per block) and T.
 hipError_t s1_panel_qr(hipStream_t st, double* A, int lda, int n, int jb,

@@ -47,6 +47,10 @@ struct paraht_ctx {
   hipEvent_t ev_aleft = nullptr;  // aux has applied a panel's left update to A
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  // a context over a caller's fixed workspace, running everything on the
+  // call's stream (the allocation-free entry point): no streams or events
+  // are created, ws is never reallocated, and larger needs are refused
+  bool single = false;
   double times[8] = {0};
   bool profiling = false;
   std::vector<hipEvent_t> pool;
@@ -88,6 +92,7 @@ int hip_fail(hipError_t e, const char* what, int line) {
 // grow-only device workspace of the context
 int ensure_ws(paraht_ctx* c, size_t bytes) {
   if (c->ws_bytes >= bytes) return 0;
+  if (c->single) return PARAHT_ERR_UNSUPPORTED;  // a fixed workspace: never reallocated
   if (c->ws) {
     CK(hipStreamSynchronize(c->stream));
     CK(hipFree(c->ws));
@@ -180,9 +185,10 @@ struct Prof {
   } while (0)
 #define PROF(kid, flops, bytes, launch) PROF_ON(c->stream, kid, flops, bytes, launch)
 
-// Creates the context's second stream (default priority) and its events.
+// Creates the context's second and third streams (default priority) and
+// their events; a single-stream context has none.
 int ensure_aux(paraht_ctx* c) {
-  if (c->aux) return 0;
+  if (c->aux || c->single) return 0;
   CK(hipEventCreateWithFlags(&c->ev_tape, hipEventDisableTiming));
   CK(hipEventCreateWithFlags(&c->ev_free[0], hipEventDisableTiming));
   CK(hipEventCreateWithFlags(&c->ev_free[1], hipEventDisableTiming));
@@ -195,12 +201,36 @@ int ensure_aux(paraht_ctx* c) {
   return 0;
 }
 
-// algorithmic flops of a list of compact-WY blocks: 4*len*k*width each
-double wy_flops(const std::vector<ApplyBlock>& v, size_t first, int cnt) {
+// The streams of a call: off-critical-path work goes to the context's second
+// (aux) and third (aux2) streams, ordered against the call's stream by
+// events; a single-stream context runs it on the call's stream, in program
+// order, and records no events.
+hipStream_t side1(paraht_ctx* c) { return c->single ? c->stream : c->aux; }
+hipStream_t side2(paraht_ctx* c) { return c->single ? c->stream : c->aux2; }
+int rec(paraht_ctx* c, hipEvent_t ev, hipStream_t s) {
+  if (c->single) return 0;
+  CK(hipEventRecord(ev, s));
+  return 0;
+}
+// s waits for the work before the latest record of ev
+int wait_on(paraht_ctx* c, hipStream_t s, hipEvent_t ev) {
+  if (c->single) return 0;
+  CK(hipStreamWaitEvent(s, ev, 0));
+  return 0;
+}
+
+// algorithmic flops of a panel's compact-WY updates: 4*len*k*width per block,
+// width = the part of [lo, hi) inside the launch's range
+double panel_wy_flops(int n, int nb, int p, int jb0, int nblocks, int which, int lo, int hi) {
   double f = 0;
-  for (int i = 0; i < cnt; ++i) {
-    const ApplyBlock& b = v[first + i];
-    f += 4.0 * b.len * b.k * std::max(0, b.hi - b.lo);
+  for (int bl = 0; bl < nblocks; ++bl) {
+    const pht::S1Block b = pht::s1_block(n, nb, p, jb0, nblocks, bl);
+    int blo = 0, bhi = n, k = b.kr;
+    if (which == 0) { blo = b.i0; k = b.k; }          // B left
+    else if (which == 1) { blo = jb0 + nb; k = b.k; } // A left
+    else if (which == 2) { k = b.k; }                 // Q
+    else if (which == 4) { bhi = b.i0 + b.len; }      // B right
+    f += 4.0 * b.len * k * std::max(0, std::min(bhi, hi) - std::max(blo, lo));
   }
   return f;
 }
@@ -308,164 +338,171 @@ struct Timer {
 };
 
 // ------------------------------------------------------------------------
+// Workspace of the two stages (nbuf = 2: tapes double-buffered between
+// streams; 1: a single stream)
+size_t s1_ws_bytes(int n, int nb, int p, int nbuf) {
+  if (n <= nb + 1) return 0;
+  const int P = p * nb;
+  const int nbmax = pht::s1_nblocks(n, nb, p, 0);
+  const int np = pht::s1_npanels(n, nb);
+  size_t nd = 0;
+  for (int q = 0; q < np; ++q) nd += pht::s1_nblocks(n, nb, p, q * nb);
+  const size_t vsz = (size_t)nbmax * P * nb, tsz = (size_t)nbmax * nb * nb;
+  const size_t scr = (size_t)P * P + (size_t)nb * P + (size_t)nbmax * 64;
+  return (2 * nbuf * vsz + 2 * nbuf * tsz + scr) * sizeof(double) + 5 * nd * sizeof(ApplyBlock) +
+         256;
+}
+size_t s2_ws_bytes(int n, int r, int nq, int nbuf) {
+  if (n <= 2) return 0;
+  const int ldr = r + nq;
+  const int nbl = (n - 1) / r + ((n - 1) % r != 0);
+  const size_t tsz = (size_t)ldr * nq * nbl;
+  const size_t msz = (size_t)ldr * ldr * nbl;
+  return (4 * nbuf * tsz + msz) * sizeof(double) + pht::s2_counter_ints(nq) * sizeof(int);
+}
+
+// ------------------------------------------------------------------------
 // Stage 1
 // ------------------------------------------------------------------------
+// Per panel (ab2rht.f90:71-467): the chain of block QRs, the left updates of
+// B (call stream), A (aux) and Q (aux), then the chain of right generators,
+// each block's reflector applied at once to rows 0..i0+len-1 of B (the
+// reference's lookahead and rest of B, ab2rht.f90:317-367, in one launch: the
+// update is latency-bound, so the rows above the lookahead cost nothing on
+// the chain).  Then the right updates of A (the rows the next panel reads on
+// the chain, the rows above on aux) and Z (aux).  The panel tapes are
+// double-buffered so that panel k's Q and Z updates overlap panel k+1.
 int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
            double* Q, int ldq, double* Z, int ldz, int nb, int p, int flags) {
   if (n <= nb + 1) return 0;  // no panel: ab2rht.f90:71 loop is empty
   hipStream_t st = c->stream;
+  if (int e = ensure_aux(c)) return e;
+  hipStream_t sa = side1(c);
   const int P = p * nb;
-  int nbmax = (n - nb) / ((p - 1) * nb);
-  if ((n - nb) % ((p - 1) * nb)) ++nbmax;
-
-  // block descriptors of every panel, uploaded once
-  struct PanelInfo {
-    int jb0, nblocks, first, i0top;
-  };
-  std::vector<PanelInfo> panels;
-  std::vector<ApplyBlock> BL, AL, QL, RR;  // RR: right blocks (B/A/Z differ only in lo/hi)
-  for (int jb = 1; jb <= n - nb - 1; jb += nb) {
-    int nblocks = (n - nb - jb + 1) / ((p - 1) * nb);
-    if ((n - nb - jb + 1) % ((p - 1) * nb)) ++nblocks;
-    PanelInfo pi{jb - 1, nblocks, (int)BL.size(), jb - 1 + nb};
-    for (int bl = 0; bl < nblocks; ++bl) {
-      const int i0 = jb - 1 + nb + (nblocks - 1 - bl) * (p - 1) * nb;
-      const int len = std::min(n - i0, P);
-      const int k = std::min(nb, len);
-      const int kr = (len > nb) ? nb : len;
-      const long long voff = (long long)bl * P * nb, toff = (long long)bl * nb * nb;
-      BL.push_back({i0, len, k, i0, n, 0, voff, toff});
-      AL.push_back({i0, len, k, jb - 1 + nb, n, 0, voff, toff});
-      QL.push_back({i0, len, k, 0, n, 0, voff, toff});
-      RR.push_back({i0, len, kr, 0, n, 0, voff, toff});
-    }
-    panels.push_back(pi);
-  }
-  const size_t nd = BL.size();
-  // B right blocks are applied to rows [0, i0+len)
-  std::vector<ApplyBlock> BR = RR;
-  for (auto& b : BR) b.hi = b.start + b.len;
-
-  // the panel tapes are double-buffered: panel k's Q and Z updates run on the
-  // second stream while panel k+1 is generated into the other buffers
+  const int nbmax = pht::s1_nblocks(n, nb, p, 0);
+  const int np = pht::s1_npanels(n, nb);
+  int nd = 0;
+  for (int q = 0; q < np; ++q) nd += pht::s1_nblocks(n, nb, p, q * nb);
+  const int nbuf = c->single ? 1 : 2;  // one stream needs no second tape
   const size_t vsz = (size_t)nbmax * P * nb, tsz = (size_t)nbmax * nb * nb;
   const size_t scr = (size_t)P * P + (size_t)nb * P + (size_t)nbmax * 64;  // + panel taus
-  const size_t dsz = 5 * nd * sizeof(ApplyBlock);
-  size_t bytes = (4 * vsz + 4 * tsz + scr) * sizeof(double) + dsz + 256;
-  if (int e = ensure_ws(c, bytes)) return e;
-  if (int e = ensure_aux(c)) return e;
-  double* VLb[2] = {(double*)c->ws, (double*)c->ws + vsz};
-  double* VRb[2] = {VLb[1] + vsz, VLb[1] + 2 * vsz};
-  double* TLb[2] = {VRb[1] + vsz, VRb[1] + vsz + tsz};
-  double* TRb[2] = {TLb[1] + tsz, TLb[1] + 2 * tsz};
-  double* scratch = TRb[1] + tsz;
-  ApplyBlock* dD = (ApplyBlock*)(((uintptr_t)(scratch + scr) + 255) & ~(uintptr_t)255);
+  if (int e = ensure_ws(c, s1_ws_bytes(n, nb, p, nbuf))) return e;
+  double* w = (double*)c->ws;
+  double *VLb[2], *VRb[2], *TLb[2], *TRb[2];
+  for (int b = 0; b < nbuf; ++b) {
+    VLb[b] = w; w += vsz;
+    VRb[b] = w; w += vsz;
+    TLb[b] = w; w += tsz;
+    TRb[b] = w; w += tsz;
+  }
+  if (nbuf == 1) {
+    VLb[1] = VLb[0]; VRb[1] = VRb[0]; TLb[1] = TLb[0]; TRb[1] = TRb[0];
+  }
+  double* gen_ws = w;  // the generators' scratch and the panel's taus
+  ApplyBlock* dD = (ApplyBlock*)(((uintptr_t)(gen_ws + scr) + 255) & ~(uintptr_t)255);
   ApplyBlock* dBL = dD;
-  ApplyBlock* dAL = dBL + nd;
-  ApplyBlock* dQL = dAL + nd;
-  ApplyBlock* dRR = dQL + nd;
-  ApplyBlock* dBR = dRR + nd;
-  CK(hipMemcpyAsync(dBL, BL.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
-  CK(hipMemcpyAsync(dAL, AL.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
-  CK(hipMemcpyAsync(dQL, QL.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
-  CK(hipMemcpyAsync(dRR, RR.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
-  CK(hipMemcpyAsync(dBR, BR.data(), nd * sizeof(ApplyBlock), hipMemcpyHostToDevice, st));
-  // the host vectors die with this frame
-  CK(hipStreamSynchronize(st));
+  ApplyBlock* dAL = dD + nd;
+  ApplyBlock* dQL = dD + 2 * (size_t)nd;
+  ApplyBlock* dRR = dD + 3 * (size_t)nd;
+  ApplyBlock* dBR = dD + 4 * (size_t)nd;
+  CK(pht::s1_block_lists(st, dD, nd, n, nb, p));
 
   const bool tc = (flags & PARAHT_TIME_COMPONENTS) != 0;
   // register-resident generators cover blocks of <= 128 rows, <= 64 reflectors
   const bool fast_gen = nb <= 64 && P <= 128 && getenv("PARAHT_GENERIC_S1") == nullptr;
   Timer tm(c, tc);
   std::vector<int> marks;
-  int kp = 0;
-  for (const PanelInfo& pi : panels) {
-    const int buf = kp & 1;
+  for (int kp = 0, jb0 = 0, first = 0; kp < np; ++kp, jb0 += nb) {
+    const int nblocks = pht::s1_nblocks(n, nb, p, jb0);
+    const int buf = kp % nbuf;
     double* VL = VLb[buf];
     double* VR = VRb[buf];
     double* TL = TLb[buf];
     double* TR = TRb[buf];
-    // panel kp-2's Q and Z updates (second stream) have read these tapes
-    if (kp >= 2) CK(hipStreamWaitEvent(st, c->ev_free[buf], 0));
+    // panel kp-2's Q and Z updates (aux) have read these tapes
+    if (kp >= 2 && nbuf == 2)
+      if (int e = wait_on(c, st, c->ev_free[buf])) return e;
     const int m0 = tm.mark();
     double qf = 0.0;
-    for (int bl = 0; bl < pi.nblocks; ++bl) qf += qr_flops(BL[pi.first + bl].len, BL[pi.first + bl].k);
+    for (int bl = 0; bl < nblocks; ++bl) {
+      const pht::S1Block b = pht::s1_block(n, nb, p, jb0, nblocks, bl);
+      qf += qr_flops(b.len, b.k);
+    }
     if (fast_gen)
       PROF(PARAHT_K_S1_PANEL_QR, qf, 0.0,
-           pht::s1_panel_qr_fast(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
+           pht::s1_panel_qr_fast(st, A, lda, n, jb0, nb, p, nblocks, VL, P,
                                  (long long)P * nb, TL, nb, (long long)nb * nb,
-                                 scratch + (size_t)P * P + (size_t)nb * P));
+                                 gen_ws + (size_t)P * P + (size_t)nb * P));
     else
       PROF(PARAHT_K_S1_PANEL_QR, qf, 0.0,
-           pht::s1_panel_qr(st, A, lda, n, pi.jb0, nb, p, pi.nblocks, VL, P,
+           pht::s1_panel_qr(st, A, lda, n, jb0, nb, p, nblocks, VL, P,
                             (long long)P * nb, TL, nb, (long long)nb * nb));
     const int m1 = tm.mark();
-    // the RQ chain reads only B: the left update of A runs beside it on the
-    // second stream, and the right update of A waits for it
-    CK(hipEventRecord(c->ev_left, st));
-    CK(hipStreamWaitEvent(c->aux, c->ev_left, 0));
-    PROF_ON(c->aux, PARAHT_K_WY_APPLY, wy_flops(AL, pi.first, pi.nblocks), 0.0,
-            pht::apply_blocks(c->aux, pht::kLeftT, A, lda, VL, P, TL, nb, dAL + pi.first,
-                              pi.nblocks, nb, P, pi.jb0 + nb, n));
-    CK(hipEventRecord(c->ev_aleft, c->aux));
-    PROF(PARAHT_K_WY_APPLY, wy_flops(BL, pi.first, pi.nblocks), 0.0,
-         pht::apply_blocks(st, pht::kLeftT, B, ldb, VL, P, TL, nb, dBL + pi.first,
-                           pi.nblocks, nb, P, pi.i0top, n));
+    // the RQ chain reads only B: the left update of A runs beside it on aux,
+    // and the right update of A waits for it
+    if (int e = rec(c, c->ev_left, st)) return e;
+    if (int e = wait_on(c, sa, c->ev_left)) return e;
+    PROF_ON(sa, PARAHT_K_WY_APPLY, panel_wy_flops(n, nb, p, jb0, nblocks, 1, 0, n), 0.0,
+            pht::apply_blocks(sa, pht::kLeftT, A, lda, VL, P, TL, nb, dAL + first, nblocks, nb,
+                              P, jb0 + nb, n));
+    if (int e = rec(c, c->ev_aleft, sa)) return e;
+    PROF(PARAHT_K_WY_APPLY, panel_wy_flops(n, nb, p, jb0, nblocks, 0, 0, n), 0.0,
+         pht::apply_blocks(st, pht::kLeftT, B, ldb, VL, P, TL, nb, dBL + first, nblocks, nb, P,
+                           jb0 + nb, n));
     const int m2 = tm.mark();
-    for (int bl = 0; bl < pi.nblocks; ++bl) {
-      const ApplyBlock& b = BR[pi.first + bl];
+    for (int bl = 0; bl < nblocks; ++bl) {
+      const pht::S1Block b = pht::s1_block(n, nb, p, jb0, nblocks, bl);
       const double rf = b.len > nb ? rq_flops(b.len, nb) : (4.0 / 3.0) * b.len * b.len * b.len;
       if (fast_gen)
         PROF(PARAHT_K_S1_RQ_GEN, rf, 0.0,
-             pht::s1_rq_fast(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
+             pht::s1_rq_fast(st, B, ldb, b.i0, b.len, nb, VR + (size_t)bl * P * nb, P,
                              TR + (size_t)bl * nb * nb, nb));
       else
         PROF(PARAHT_K_S1_RQ_GEN, rf, 0.0,
-             pht::s1_rq_gen(st, B, ldb, b.start, b.len, nb, VR + (size_t)bl * P * nb, P,
-                            TR + (size_t)bl * nb * nb, nb, scratch));
+             pht::s1_rq_gen(st, B, ldb, b.i0, b.len, nb, VR + (size_t)bl * P * nb, P,
+                            TR + (size_t)bl * nb * nb, nb, gen_ws));
       // the subdiagonal part of the block's first columns becomes exact
       // zeros inside the update where the applier can (ab2rht.f90:329-333)
       const int zc = b.len > nb ? nb : b.len;
-      const bool fuse = pht::apply_fuses_zero(b.k, P);
-      PROF(PARAHT_K_WY_APPLY, wy_flops(BR, pi.first + bl, 1), 0.0,
-           pht::apply_blocks(st, pht::kRightN, B, ldb, VR, P, TR, nb,
-                             dBR + pi.first + bl, 1, b.k, P, 0, b.start + b.len,
-                             fuse ? zc : 0));
+      const bool fuse = pht::apply_fuses_zero(b.kr, P);
+      PROF(PARAHT_K_WY_APPLY, 4.0 * b.len * b.kr * (b.i0 + b.len), 0.0,
+           pht::apply_blocks(st, pht::kRightN, B, ldb, VR, P, TR, nb, dBR + first + bl, 1,
+                             b.kr, P, 0, b.i0 + b.len, fuse ? zc : 0));
       if (!fuse)
-        PROF(PARAHT_K_OTHER, 0.0, 0.0, pht::zero_below(st, B, ldb, b.start, b.len, zc));
+        PROF(PARAHT_K_OTHER, 0.0, 0.0, pht::zero_below(st, B, ldb, b.i0, b.len, zc));
     }
     const int m3 = tm.mark();
-    // this panel's right tapes are complete: the second stream takes the rows
-    // of A above the next panel's first row (jb0 + nb), which no later panel
-    // generation reads and no later main-stream update touches before the
-    // next panel's tapes exist, then Q and Z (written by these updates only)
-    CK(hipEventRecord(c->ev_tape, st));
-    CK(hipStreamWaitEvent(c->aux, c->ev_tape, 0));
-    const int split = std::min(n, pi.jb0 + nb);
-    const double fa = wy_flops(RR, pi.first, pi.nblocks);
+    // this panel's right tapes are complete: aux takes the rows of A above
+    // the next panel's first row (jb0 + nb), which no later panel generation
+    // reads and no later call-stream update touches before the next panel's
+    // tapes exist, then Q and Z (written by these updates only)
+    if (int e = rec(c, c->ev_tape, st)) return e;
+    if (int e = wait_on(c, sa, c->ev_tape)) return e;
+    const int split = std::min(n, jb0 + nb);
     if (split > 0)
-      PROF_ON(c->aux, PARAHT_K_WY_APPLY, fa * split / n, 0.0,
-              pht::apply_blocks(c->aux, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
-                                pi.nblocks, nb, P, 0, split));
+      PROF_ON(sa, PARAHT_K_WY_APPLY, panel_wy_flops(n, nb, p, jb0, nblocks, 3, 0, split), 0.0,
+              pht::apply_blocks(sa, pht::kRightN, A, lda, VR, P, TR, nb, dRR + first, nblocks,
+                                nb, P, 0, split));
     // the rows the next panel's QR reads stay on the chain, after this
-    // panel's left update of A (second stream)
-    CK(hipStreamWaitEvent(st, c->ev_aleft, 0));
-    PROF(PARAHT_K_WY_APPLY, fa * (n - split) / n, 0.0,
-         pht::apply_blocks(st, pht::kRightN, A, lda, VR, P, TR, nb, dRR + pi.first,
-                           pi.nblocks, nb, P, split, n));
+    // panel's left update of A (aux)
+    if (int e = wait_on(c, st, c->ev_aleft)) return e;
+    PROF(PARAHT_K_WY_APPLY, panel_wy_flops(n, nb, p, jb0, nblocks, 3, split, n), 0.0,
+         pht::apply_blocks(st, pht::kRightN, A, lda, VR, P, TR, nb, dRR + first, nblocks, nb, P,
+                           split, n));
     const int m4 = tm.mark();
-    PROF_ON(c->aux, PARAHT_K_WY_APPLY, wy_flops(QL, pi.first, pi.nblocks), 0.0,
-            pht::apply_blocks(c->aux, pht::kRightN, Q, ldq, VL, P, TL, nb, dQL + pi.first,
-                              pi.nblocks, nb, P, 0, n));
-    PROF_ON(c->aux, PARAHT_K_WY_APPLY, wy_flops(RR, pi.first, pi.nblocks), 0.0,
-            pht::apply_blocks(c->aux, pht::kRightN, Z, ldz, VR, P, TR, nb, dRR + pi.first,
-                              pi.nblocks, nb, P, 0, n));
-    CK(hipEventRecord(c->ev_free[buf], c->aux));
-    marks.insert(marks.end(), {m0, m1, m2, m3, m4});
-    ++kp;
+    PROF_ON(sa, PARAHT_K_WY_APPLY, panel_wy_flops(n, nb, p, jb0, nblocks, 2, 0, n), 0.0,
+            pht::apply_blocks(sa, pht::kRightN, Q, ldq, VL, P, TL, nb, dQL + first, nblocks, nb,
+                              P, 0, n));
+    PROF_ON(sa, PARAHT_K_WY_APPLY, panel_wy_flops(n, nb, p, jb0, nblocks, 3, 0, n), 0.0,
+            pht::apply_blocks(sa, pht::kRightN, Z, ldz, VR, P, TR, nb, dRR + first, nblocks, nb,
+                              P, 0, n));
+    if (int e = rec(c, c->ev_free[buf], sa)) return e;
+    if (tc) marks.insert(marks.end(), {m0, m1, m2, m3, m4});
+    first += nblocks;
   }
-  // join: the last panel's Q and Z updates (the second stream runs in order)
-  if (kp > 0) CK(hipStreamWaitEvent(st, c->ev_free[(kp - 1) & 1], 0));
+  // join: the last panel's Q and Z updates (aux runs its updates in order)
+  if (np > 0)
+    if (int e = wait_on(c, st, c->ev_free[(np - 1) % nbuf])) return e;
   if (tc) {
     CK(hipStreamSynchronize(st));
     double gl = 0, gr = 0, mul = 0;
@@ -553,13 +590,15 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   const size_t msz = (size_t)ldr * ldr * nbl;
   // two tape buffers: the Q/Z updates (and the A/B rest) of sweep k run on
   // the other streams while sweep k+1 is generated into the other buffer
-  if (int e = ensure_ws(c, (8 * tsz + msz) * sizeof(double) + pht::s2_counter_ints(nq) * sizeof(int)))
-    return e;
+  // (one stream needs one)
+  const int nbuf = c->single ? 1 : 2;
+  if (int e = ensure_ws(c, s2_ws_bytes(n, r, nq, nbuf))) return e;
   if (int e = ensure_aux(c)) return e;
-  double* tape[2] = {(double*)c->ws, (double*)c->ws + 4 * tsz};
-  double* Mws = tape[1] + 4 * tsz;
+  double* tape[2] = {(double*)c->ws, (double*)c->ws + (nbuf - 1) * 4 * tsz};
+  double* Mws = (double*)c->ws + nbuf * 4 * tsz;
+  hipStream_t sq = side2(c);  // Q and Z
   const bool tc = (flags & PARAHT_TIME_COMPONENTS) != 0;
-  const int look = s2_lookahead();
+  const int look = c->single ? 0 : s2_lookahead();
   const bool split = pht::s2_far_split(r, nq) && look != 0;
   hipStream_t rest = look == 2 ? st : c->aux;  // the stream of the A/B rest
   const int Lc = s2_look_cols(nq), Lr = s2_look_rows(nq);
@@ -569,8 +608,8 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
   bool far_pending = false;
   // aux2 starts after everything enqueued on the call's stream so far
   // (stage 1's updates of Q and Z included)
-  CK(hipEventRecord(c->ev_tape, st));
-  CK(hipStreamWaitEvent(c->aux2, c->ev_tape, 0));
+  if (int e = rec(c, c->ev_tape, st)) return e;
+  if (int e = wait_on(c, sq, c->ev_tape)) return e;
   for (int s = 1, k = 0; s <= n - 2; s += nq, ++k) {
     buf = k & 1;
     double* CU = tape[buf];
@@ -597,20 +636,21 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
     const int m0 = tm.mark();
     // the buffer's previous tape (sweep k-2) must be consumed by its Q/Z update
     // (its A/B rest finished before sweep k-1's lookahead update)
-    if (k >= 2) CK(hipStreamWaitEvent(st, c->ev_free[buf], 0));
+    if (k >= 2)
+      if (int e = wait_on(c, st, c->ev_free[buf])) return e;
     PROF(PARAHT_K_OTHER, 0.0, 0.0, pht::s2_prepare(st, P, CU, SU, CV, SV));
     PROF(PARAHT_K_S2_GEN, s2_gen_flops(n, r, nq, s), 0.0,
          pht::s2_generate(st, A, lda, B, ldb, P, CU, SU, CV, SV));
     const int m1 = tm.mark();
     const double w = (double)(n - s);
-    CK(hipEventRecord(c->ev_tape, st));
-    CK(hipStreamWaitEvent(c->aux2, c->ev_tape, 0));
+    if (int e = rec(c, c->ev_tape, st)) return e;
+    if (int e = wait_on(c, sq, c->ev_tape)) return e;
     // algorithmic traffic: every element of the sweep's columns s+1..n of Q
     // and Z is read and written once
-    PROF_ON(c->aux2, PARAHT_K_S2_ROWS, 0.0, 16.0 * w * 2.0 * n,
-            pht::s2_apply_rows(c->aux2, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
+    PROF_ON(sq, PARAHT_K_S2_ROWS, 0.0, 16.0 * w * 2.0 * n,
+            pht::s2_apply_rows(sq, A, lda, B, ldb, Q, ldq, Z, ldz, P, CU, SU, CV, SV,
                                pht::kRowsQZ));
-    CK(hipEventRecord(c->ev_free[buf], c->aux2));
+    if (int e = rec(c, c->ev_free[buf], sq)) return e;
     const int rows_all = std::min(n, s + (P.nblocks - 1) * r);  // rows above some window
     const int col0 = std::min(n, s + nq - 1);                  // first column right of one
     if (split) {
@@ -655,12 +695,12 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
            pht::s2_apply_cols(st, A, lda, B, ldb, P, CU, SU, allc));
     }
     const int m2 = tm.mark();
-    marks.insert(marks.end(), {m0, m1, m2});
+    if (tc) marks.insert(marks.end(), {m0, m1, m2});
   }
   // join: the last A/B rest and the last Q/Z update (each stream runs its
   // updates in order)
   if (far_pending) CK(hipStreamWaitEvent(st, c->ev_far, 0));
-  CK(hipStreamWaitEvent(st, c->ev_free[buf], 0));
+  if (int e = wait_on(c, st, c->ev_free[buf])) return e;
   {
     int err = 0;
     CK(hipMemcpyAsync(&err, (int*)(Mws + msz) + (size_t)nq * (1 + pht::kS2MaxHelpers),

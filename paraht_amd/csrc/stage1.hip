@@ -310,6 +310,33 @@ __global__ void k_zero_below(double* B, int ldb, int i0, int len, int ncols) {
     B[PHT_IDX(r, j, ldb)] = 0.0;
 }
 
+// One workgroup per panel: the panel's blocks, after the blocks of the
+// panels before it.
+__global__ void k_s1_block_lists(ApplyBlock* __restrict__ lists, int nd, int n, int nb, int p) {
+  const int panel = blockIdx.x, jb0 = panel * nb;
+  int first = 0;
+  for (int q = 0; q < panel; ++q) first += s1_nblocks(n, nb, p, q * nb);
+  const int nblocks = s1_nblocks(n, nb, p, jb0);
+  const int P = p * nb;
+  for (int bl = threadIdx.x; bl < nblocks; bl += blockDim.x) {
+    const S1Block b = s1_block(n, nb, p, jb0, nblocks, bl);
+    const long long voff = (long long)bl * P * nb, toff = (long long)bl * nb * nb;
+    const int e = first + bl;
+    lists[e] = {b.i0, b.len, b.k, b.i0, n, 0, voff, toff};                   // B left
+    lists[nd + e] = {b.i0, b.len, b.k, jb0 + nb, n, 0, voff, toff};          // A left
+    lists[2 * nd + e] = {b.i0, b.len, b.k, 0, n, 0, voff, toff};             // Q
+    lists[3 * nd + e] = {b.i0, b.len, b.kr, 0, n, 0, voff, toff};            // A, Z right
+    lists[4 * nd + e] = {b.i0, b.len, b.kr, 0, b.i0 + b.len, 0, voff, toff}; // B right
+  }
+}
+
+hipError_t s1_block_lists(hipStream_t st, ApplyBlock* lists, int nd, int n, int nb, int p) {
+  const int np = s1_npanels(n, nb);
+  if (np <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_s1_block_lists, dim3(np), dim3(128), 0, st, lists, nd, n, nb, p);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 static constexpr size_t kMaxLds = 160 * 1024;
 
