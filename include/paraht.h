@@ -114,6 +114,24 @@ int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
                       int ldb, double* dQ, int ldq, double* dZ, int ldz, int r,
                       int p, int nq, int flags);
 
+/* ---- Allocation-free reduction ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the two stages of paraht_reduce_dev, on the caller's
+ * stream (a hipStream_t as void*, NULL for the null stream) and with no
+ * hipMalloc, hipFree, stream or event of their own.  The workspace is a
+ * 64 MiB array in the library's device data, which each GPU the library runs
+ * on holds until the process ends (r = 64, p = 2 fits n <= 19392; r = 16,
+ * p = 8 fits n <= 35120);
+ * larger pencils return PARAHT_ERR_UNSUPPORTED before any work.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU race.  flags
+ * may hold PARAHT_STAGE2_GIVENS only (per-phase timing needs events).
+ * Arguments as paraht_reduce_dev, the context replaced by the stream; the
+ * call waits for the stream. */
+int paraht_reduce_dev_noalloc(void* stream, int n, double* dA, int lda, double* dB, int ldb,
+                              double* dQ, int ldq, double* dZ, int ldz, int r, int p, int nq,
+                              int flags);
+
 /* ---- Batched reduction of many small pencils ----
  * No counterpart in the reference.  Reduces `count` independent pencils with
  * both stages in one persistent launch: one workgroup per compute unit takes

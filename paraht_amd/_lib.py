@@ -45,6 +45,7 @@ SIGNATURES = {
     "paraht_ab2rht_dev": ([_P] + _SIG_HOST12, _I),
     "paraht_rht2ht_dev": ([_P] + _SIG_HOST12, _I),
     "paraht_reduce_dev": ([_P] + _SIG_HOST12 + [_I], _I),
+    "paraht_reduce_dev_noalloc": ([_P] + _SIG_HOST12 + [_I], _I),
     "paraht_batched_dev": ([_P, _I, ctypes.POINTER(Pencil), _DP, _DP, _DP, _DP, _I], _I),
     "paraht_batched_defaults": ([_I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                  ctypes.POINTER(_I)], _I),

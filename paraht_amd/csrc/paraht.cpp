@@ -975,6 +975,46 @@ int adjoint(paraht_ctx* c, int count, const paraht_pencil* pencils, const double
 }  // namespace
 
 // ------------------------------------------------------------------------
+// Allocation-free reduction
+// ------------------------------------------------------------------------
+// The workspace of paraht_reduce_dev_noalloc: kScratchDoubles FP64 words
+// (64 MiB) per device in the library's device data, which each GPU the
+// library runs on holds until the process ends.  Host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device
+// (the array's name is a host-side stand-in there).
+constexpr size_t kScratchDoubles = (size_t)8 << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+struct ScratchDev {
+  bool ready = false;
+  double* ptr = nullptr;  // the device's copy of `scratch`
+  int cu_count = 0;
+};
+ScratchDev g_scratch[64];
+std::mutex g_scratch_mu;
+
+// Once per device, with the device current: checks the architecture and
+// keeps the address of the device's copy of the scratch array.
+int scratch_for_device(int dev, ScratchDev* out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  ScratchDev& d = g_scratch[dev];
+  if (!d.ready) {
+    hipDeviceProp_t prop;
+    CK(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PARAHT_ERR_NODEV;
+    void* p = nullptr;
+    CK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    d.ptr = (double*)p;
+    d.cu_count = prop.multiProcessorCount;
+    d.ready = true;
+  }
+  *out = d;
+  return 0;
+}
+}  // namespace
+
+// ------------------------------------------------------------------------
 // C-ABI
 // ------------------------------------------------------------------------
 extern "C" {
@@ -1135,6 +1175,45 @@ int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
   if (!pht::s1_supported(r, p)) return PARAHT_ERR_UNSUPPORTED;
   if (int e = s2_check(r, nq)) return e;  // before stage 1 changes anything
   return run_dev(ctx, kBoth, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, p, nq, flags);
+}
+
+int paraht_reduce_dev_noalloc(void* stream, int n, double* dA, int lda, double* dB, int ldb,
+                              double* dQ, int ldq, double* dZ, int ldz, int r, int p, int nq,
+                              int flags) {
+  if (int e = check_common(n, lda, ldb, ldq, ldz)) return e - 1;
+  if (r < 1) return -11;
+  if (p < 2) return -12;
+  if (nq < 1) return -13;
+  if (flags & ~PARAHT_STAGE2_GIVENS) return -14;  // per-phase timing needs events
+  if (!pht::s1_supported(r, p)) return PARAHT_ERR_UNSUPPORTED;
+  if (int e = s2_check(r, nq)) return e;
+  const size_t need = std::max(s1_ws_bytes(n, r, p, 1), s2_ws_bytes(n, r, nq, 1));
+  if (need > kScratchDoubles * sizeof(double)) {
+    if (verbose())
+      fprintf(stderr, "paraht: n=%d needs %zu bytes of workspace, the allocation-free path has %zu\n",
+              n, need, kScratchDoubles * sizeof(double));
+    return PARAHT_ERR_UNSUPPORTED;
+  }
+  int dev = 0, ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PARAHT_ERR_NODEV;
+  CK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return PARAHT_ERR_NODEV;
+  ScratchDev sd;
+  if (int e = scratch_for_device(dev, &sd)) return e;
+  if (n == 0) return 0;
+  // a context on the stack over the scratch array: the call's stream only,
+  // no streams, events or allocations of its own
+  paraht_ctx c;
+  c.device = dev;
+  c.cu_count = sd.cu_count;
+  c.stream = (hipStream_t)stream;
+  c.single = true;
+  c.ws = sd.ptr;
+  c.ws_bytes = kScratchDoubles * sizeof(double);
+  int st = stage1(&c, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, p, flags);
+  if (!st) st = stage2(&c, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, nq, flags);
+  if (!st) CK(hipStreamSynchronize(c.stream));
+  return st;
 }
 
 int paraht_batched_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils, double* dA,
