@@ -166,6 +166,38 @@ def dgghd3(A0, B0):
     return A, B, Q, Z
 
 
+def dgghd3_mkl(A0, B0):
+    """The same DGGHD3 call through Intel MKL (libmkl_rt), when the host has
+    it (the reference builds against MKL, make.inc.example:12-16); None
+    otherwise.  Used only as a timed CPU baseline."""
+    cands = [os.path.join(p, "libmkl_rt.so") for p in
+             (os.environ.get("MKLROOT", ""), "/opt/conda/lib")]
+    path = next((c for c in cands if c and os.path.exists(c)), None)
+    if path is None:
+        return None
+    f = ctypes.CDLL(path).dgghd3_
+    n = A0.shape[0]
+    A = np.array(A0, dtype=np.float64, order="F")
+    B = np.array(B0, dtype=np.float64, order="F")
+    Q = np.zeros((n, n), order="F")
+    Z = np.zeros((n, n), order="F")
+    ci = ctypes.c_int
+    nn, ilo, ihi, ld, info = ci(n), ci(1), ci(n), ci(max(1, n)), ci(0)
+    cq = ctypes.c_char(b"I")
+    wq = np.zeros(1)
+    for lwork in (ci(-1), None):
+        work = wq if lwork is not None else np.zeros(max(1, int(wq[0])))
+        lw = lwork if lwork is not None else ci(work.size)
+        f(ctypes.byref(cq), ctypes.byref(cq), ctypes.byref(nn), ctypes.byref(ilo),
+          ctypes.byref(ihi), _ptr(A), ctypes.byref(ld), _ptr(B), ctypes.byref(ld),
+          _ptr(Q), ctypes.byref(ld), _ptr(Z), ctypes.byref(ld),
+          work.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(lw),
+          ctypes.byref(info))
+    if info.value != 0:
+        raise RuntimeError(f"mkl dgghd3 info={info.value}")
+    return A, B, Q, Z
+
+
 # --------------------------------------------------------------------------
 # Metrics
 # --------------------------------------------------------------------------
