@@ -257,8 +257,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=8000)
     ap.add_argument("--r", type=int, default=64)
-    ap.add_argument("--p", type=int, default=2)
-    ap.add_argument("--nq", type=int, default=16)
+    ap.add_argument("--p", type=int, default=0,
+                    help="blocks per stage-1 reflector block (0: the tuned table, else 2)")
+    ap.add_argument("--nq", type=int, default=0,
+                    help="columns per stage-2 sweep (0: the tuned table, else 16)")
     ap.add_argument("--cpu-n", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -277,6 +279,12 @@ def main():
     import torch
 
     import paraht_amd as ph
+    from paraht_amd.tune import tuned_params
+    tuned = tuned_params(args.n, args.r)
+    if not args.p:
+        args.p = tuned[0] if tuned else 2
+    if not args.nq:
+        args.nq = tuned[1] if tuned else 16
     n, r, p, nq = args.n, args.r, args.p, args.nq
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
@@ -346,7 +354,8 @@ def main():
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"one FP64 pencil n={n}, r={r}, p={p}, nq={nq}; "
                                "A~N(0,1), B=R of QR of N(0,1), Q=Z=I",
-                   "n": n, "r": r, "p": p, "nq": nq, "stage2": "givens"},
+                   "n": n, "r": r, "p": p, "nq": nq, "stage2": "givens",
+                   "p_nq_from": "tuned table paraht_amd/tuned.json" if tuned else "fixed"},
         "gflops": flops / t_red / 1e9,
         "pct_fp64_peak": 100.0 * flops / t_red / FP64_PEAK,
         "stage1_s": t1, "stage2_s": t2,
