@@ -1,19 +1,27 @@
 ! Benchmark driver with the command line and result line of the reference's
 ! src/profilereduction.f90:
-!   profilereduction n algorithm [r p nq [time_components]]
+!   profilereduction n algorithm [r p nq [time_components]] [pencil=FILE] [out=FILE]
 ! algorithm 1: ab2rht_simple + rht2ht_householder (modules ab2rht / rht2ht,
 ! here the GPU drop-ins); algorithm 2: LAPACK DGGHD3 on the host.  The pencil
 ! is A uniform on [0,1), B the R factor of a uniform [0,1) matrix, Q = Z = I,
-! from the compiler's random_number seeded with 1302.  Prints
+! from the compiler's random_number seeded with 1302, or read from a pencil
+! file (pencil=FILE; n = 0 takes the file's order).  out=FILE writes the
+! reduced H, T, Q, Z.  Prints
 !   n: ;runtime: ;error: ||Q^T A0 Z - H||/||A0|| ||Q^T B0 Z - T||/||B0||
 ! (and for algorithm 1 the two phase times and r, p, nq).
+!
+! Pencil files (paraht_amd.pencils.save_pencil / load_pencil): the 8 bytes
+! "PHTPENC1", the order n as a little-endian 64-bit integer, then A and B,
+! each n x n column-major little-endian FP64.  Output files: "PHTREDU1", n,
+! then H, T, Q, Z in the same layout.
 program profilereduction
    use ab2rht
    use rht2ht
    implicit none
 
-   integer :: n, algorithm, r, p, nq, flagarg, info, nseed, k
+   integer :: n, algorithm, r, p, nq, flagarg, info, nseed, k, npos, iarg
    logical :: time_components
+   character(len=4096) :: pencil_file, out_file, full
    double precision, allocatable :: A(:, :), B(:, :), Q(:, :), Z(:, :), A0(:, :), B0(:, :), W(:, :)
    double precision, allocatable :: tau(:), work(:)
    integer, allocatable :: seed(:)
@@ -24,39 +32,58 @@ program profilereduction
 
    r = 0; p = 0; nq = 0
    time_components = .false.
-   if (command_argument_count() < 2) then
-      write (*, *) 'usage: profilereduction n algorithm (r p nq time_components)'
+   pencil_file = ''
+   out_file = ''
+   ! key=value options may follow the positional arguments
+   npos = 0
+   do iarg = 1, command_argument_count()
+      call get_command_argument(iarg, full)
+      if (full(1:7) == 'pencil=') then
+         pencil_file = full(8:)
+      else if (full(1:4) == 'out=') then
+         out_file = full(5:)
+      else
+         npos = npos + 1
+      end if
+   end do
+   if (npos < 2) then
+      write (*, *) 'usage: profilereduction n algorithm (r p nq time_components) (pencil=FILE) (out=FILE)'
       stop
    end if
    call get_command_argument(1, arg); read (arg, *) n
    call get_command_argument(2, arg); read (arg, *) algorithm
    if (algorithm == 1) then
-      if (command_argument_count() < 5) then
+      if (npos < 5) then
          write (*, *) 'algorithm 1 needs the parameters r, p and nq'
          stop
       end if
       call get_command_argument(3, arg); read (arg, *) r
       call get_command_argument(4, arg); read (arg, *) p
       call get_command_argument(5, arg); read (arg, *) nq
-      if (command_argument_count() >= 6) then
+      if (npos >= 6) then
          call get_command_argument(6, arg); read (arg, *) flagarg
          time_components = flagarg == 1
       end if
    end if
+   if (len_trim(pencil_file) > 0) n = pencil_order(trim(pencil_file), n)
 
    allocate (A(n, n), B(n, n), Q(n, n), Z(n, n), A0(n, n), B0(n, n), W(n, n))
    allocate (tau(n), work(max(1, 64*n)))
-   call random_seed(size=nseed)
-   allocate (seed(nseed))
-   seed = 0
-   seed(1) = 1302
-   call random_seed(put=seed)
-   call random_number(A)
-   call random_number(B)
-   call dgeqrf(n, n, B, n, tau, work, size(work), info)
-   do k = 1, n - 1
-      B(k + 1:n, k) = 0.0d0
-   end do
+   if (len_trim(pencil_file) > 0) then
+      call read_pencil(trim(pencil_file), n, A, B)
+   else
+      call random_seed(size=nseed)
+      allocate (seed(nseed))
+      seed = 0
+      seed(1) = 1302
+      call random_seed(put=seed)
+      call random_number(A)
+      call random_number(B)
+      call dgeqrf(n, n, B, n, tau, work, size(work), info)
+      do k = 1, n - 1
+         B(k + 1:n, k) = 0.0d0
+      end do
+   end if
    call identity(Q)
    call identity(Z)
    A0 = A
@@ -82,6 +109,7 @@ program profilereduction
 
    ea = residual(A0, A)
    eb = residual(B0, B)
+   if (len_trim(out_file) > 0) call write_reduced(trim(out_file), n, A, B, Q, Z)
    if (algorithm == 1) then
       write (*, '("n:",I6," ;runtime:",F16.6," ;error:",2ES16.6, " ;phase 1 time:",F16.6, &
              &" ;phase 2 time:",F16.6," ;r:",I4," ;p:",I4," ;nq:",I4)') n, total, ea, eb, t1, t2, r, p, nq
@@ -90,6 +118,44 @@ program profilereduction
    end if
 
 contains
+
+   ! The order of a pencil file (checked against a nonzero n from the
+   ! command line).
+   integer function pencil_order(path, n_arg)
+      character(len=*), intent(in) :: path
+      integer, intent(in) :: n_arg
+      character(len=8) :: magic
+      integer(8) :: n8
+      integer :: u
+      open (newunit=u, file=path, access='stream', form='unformatted', status='old', action='read')
+      read (u) magic, n8
+      close (u)
+      if (magic /= 'PHTPENC1') error stop 'profilereduction: not a pencil file'
+      if (n_arg /= 0 .and. n_arg /= int(n8)) error stop 'profilereduction: n differs from the file'
+      pencil_order = int(n8)
+   end function
+
+   subroutine read_pencil(path, n, A, B)
+      character(len=*), intent(in) :: path
+      integer, intent(in) :: n
+      double precision, intent(out) :: A(n, n), B(n, n)
+      character(len=8) :: magic
+      integer(8) :: n8
+      integer :: u
+      open (newunit=u, file=path, access='stream', form='unformatted', status='old', action='read')
+      read (u) magic, n8, A, B
+      close (u)
+   end subroutine
+
+   subroutine write_reduced(path, n, H, T, Q, Z)
+      character(len=*), intent(in) :: path
+      integer, intent(in) :: n
+      double precision, intent(in) :: H(n, n), T(n, n), Q(n, n), Z(n, n)
+      integer :: u
+      open (newunit=u, file=path, access='stream', form='unformatted', status='replace', action='write')
+      write (u) 'PHTREDU1', int(n, 8), H, T, Q, Z
+      close (u)
+   end subroutine
 
    subroutine identity(M)
       double precision, intent(out) :: M(:, :)

@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from paraht_amd import dropin
@@ -60,3 +61,49 @@ def test_driver_without_device_fails_loudly(driver):
                          text=True, timeout=120)
     assert out.returncode != 0
     assert "ab2rht_simple: paraht status 3" in out.stdout
+
+
+def test_pencil_file_round_trip(tmp_path):
+    """save_pencil / load_pencil (and the reduced-pencil pair) keep every bit."""
+    from paraht_amd.pencils import (load_pencil, load_reduced, random_pencil, save_pencil,
+                                    save_reduced)
+    A, B = random_pencil(37, seed=5)
+    f = tmp_path / "p.pht"
+    save_pencil(f, A, B)
+    A2, B2 = load_pencil(f)
+    assert np.array_equal(A, A2) and np.array_equal(B, B2)
+    assert A2.flags.f_contiguous
+    g = tmp_path / "r.pht"
+    save_reduced(g, A, B, B, A)
+    assert all(np.array_equal(x, y) for x, y in zip(load_reduced(g), (A, B, B, A)))
+    with pytest.raises(ValueError):
+        load_reduced(f)  # a pencil is not a reduced pencil
+    with pytest.raises(ValueError):
+        save_pencil(f, A, B[:5, :5])
+
+
+def test_driver_reads_and_writes_pencil_files(driver, tmp_path, oracle_mod):
+    """The driver reduces an external pencil (pencil=FILE, n = 0 takes the
+    file's order) and writes H, T, Q, Z (out=FILE): algorithm 2 (host DGGHD3)
+    on the CPU, checked against the oracle's DGGHD3 after sign normalisation
+    and against the pencil in the file."""
+    from paraht_amd.pencils import illcond_pencil, load_reduced, save_pencil
+    O = oracle_mod
+    n = 90
+    A0, B0 = illcond_pencil(n, seed=3, zero_frac=0.0)
+    f, g = tmp_path / "in.pht", tmp_path / "out.pht"
+    save_pencil(f, A0, B0)
+    out = subprocess.run([driver, "0", "2", f"pencil={f}", f"out={g}"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    m = LINE.search(out.stdout)
+    assert m and int(m.group(1)) == n, out.stdout
+    H, T, Q, Z = load_reduced(g)
+    ea, eb = O.backward_errors(A0, B0, H, T, Q, Z)
+    u = 2.0 ** -53
+    assert ea <= 10 * n * u and eb <= 10 * n * u
+    assert abs(ea - float(m.group(3))) <= 1e-3 * max(ea, 1e-300) + 1e-30
+    assert O.below_band(H, 1) == 0.0 and O.below_band(T, 0) == 0.0
+    bad = subprocess.run([driver, "91", "2", f"pencil={f}"], capture_output=True, text=True,
+                         timeout=60)
+    assert bad.returncode != 0 and "n differs" in (bad.stdout + bad.stderr)

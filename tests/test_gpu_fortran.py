@@ -33,3 +33,27 @@ def test_driver_time_components():
     assert out.returncode == 0, out.stdout + out.stderr
     assert "genlefttime" in out.stdout and "gen:" in out.stdout, out.stdout
     assert LINE.search(out.stdout), out.stdout
+
+
+def test_driver_paraht_on_a_pencil_file(tmp_path):
+    """Algorithm 1 (both GPU stages through the modules) on an external pencil
+    read from a file, the reduced pencil written back and checked."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle"))
+    import oracle as O
+    from paraht_amd.pencils import illcond_pencil, load_reduced, save_pencil
+    n, r, p, nq = 200, 16, 2, 8
+    A0, B0 = illcond_pencil(n, seed=8)
+    f, g = tmp_path / "in.pht", tmp_path / "out.pht"
+    save_pencil(f, A0, B0)
+    out = subprocess.run([dropin.DRIVER, "0", "1", str(r), str(p), str(nq), f"pencil={f}",
+                          f"out={g}"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    m = LINE.search(out.stdout)
+    assert m and int(m.group(1)) == n, out.stdout
+    H, T, Q, Z = load_reduced(g)
+    ea, eb = O.backward_errors(A0, B0, H, T, Q, Z)
+    u = 2.0 ** -53
+    assert ea <= 10 * n * u and eb <= 10 * n * u, (ea, eb)
+    assert O.below_band(H, 1) == 0.0 and O.below_band(T, 0) == 0.0
