@@ -114,6 +114,21 @@ int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
                       int ldb, double* dQ, int ldq, double* dZ, int ldz, int r,
                       int p, int nq, int flags);
 
+/* ---- QR of a general B (before stage 1) ----
+ * The reference's driver makes B upper triangular with LAPACK dgeqrf on the
+ * host before the reduction and keeps only R (src/profilereduction.f90:117-123;
+ * its A is random).  These do it on the GPU and carry the orthogonal factor:
+ * B <- R = Q_B^T B, A <- Q_B^T A, Q <- Q Q_B, so that a later reduction of
+ * (A, B) with this Q gives Q^T A0 Z = H, Q^T B0 Z = T for the original pencil
+ * (A0, B0).  nb: the columns per panel, p: panel blocks of p*nb rows (as for
+ * stage 1; nb <= 64 and p*nb <= 128 take the register-resident generators).
+ * Z is not involved.  Returns -k for an illegal k-th argument,
+ * PARAHT_ERR_UNSUPPORTED where the generators cannot take (nb, p). */
+int paraht_qr_b(int n, double* A, int lda, double* B, int ldb, double* Q,
+                int ldq, int nb, int p);
+int paraht_qr_b_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
+                    int ldb, double* dQ, int ldq, int nb, int p);
+
 /* ---- Allocation-free reduction ----
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

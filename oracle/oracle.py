@@ -14,6 +14,9 @@ Contents
 * LAPACK DGGHD3 from the host's LAPACK (scipy's OpenBLAS) -- an independent
   library routine that produces the same Hessenberg-triangular form up to
   diagonal signs (generalized implicit-Q theorem), used to pin the oracle;
+* ``qr_b``: B <- R of LAPACK DGEQRF with exact zeros below the diagonal
+  (src/profilereduction.f90:117-123), the Householder factor carried into A
+  and Q (DORMQR, DORGQR), as LAPACK's DGGES does before DGGHD3;
 * the reference's result metrics (src/profilereduction.f90:164-174) plus the
   checks the reference lacks: orthogonality, exact structural zeros, sign
   normalisation and generalized eigenvalues in the chordal metric.
@@ -164,6 +167,28 @@ def dgghd3(A0, B0):
     if info.value != 0:
         raise RuntimeError(f"dgghd3 info={info.value}")
     return A, B, Q, Z
+
+
+def qr_b(A0, B0, Q0=None):
+    """QR of a general B through the host's LAPACK (scipy): B0 = Q_B R with
+    DGEQRF; returns (Q_B^T A0, R with exact zeros below the diagonal,
+    Q0 Q_B).  The reference's driver keeps only R (its A is random:
+    src/profilereduction.f90:117-123); carrying Q_B into A and Q keeps
+    (A, B) equivalent to the input pencil."""
+    from scipy.linalg import lapack
+    n = B0.shape[0]
+    A = np.array(A0, dtype=np.float64, order="F")
+    Q = np.eye(n, order="F") if Q0 is None else np.array(Q0, dtype=np.float64, order="F")
+    if n == 0:
+        return A, np.array(B0, dtype=np.float64, order="F"), Q
+    qr, tau, _, info = lapack.dgeqrf(np.array(B0, dtype=np.float64, order="F"))
+    assert info == 0
+    R = np.triu(qr)
+    A, _, info = lapack.dormqr("L", "T", qr, tau, A, max(1, n))
+    assert info == 0
+    Q, _, info = lapack.dormqr("R", "N", qr, tau, Q, max(1, n))
+    assert info == 0
+    return np.asfortranarray(A), np.asfortranarray(R), np.asfortranarray(Q)
 
 
 def dgghd3_mkl(A0, B0):

@@ -30,7 +30,8 @@ from ._lib import PARAHT_STAGE2_GIVENS, PARAHT_TIME_COMPONENTS, ParahtError
 
 __all__ = [
     "ab2rht_simple", "rht2ht_householder", "rht2ht_givens",
-    "rht2ht_householder_unblocked", "reduce", "Context", "ParahtError",
+    "rht2ht_householder_unblocked", "reduce", "qr_b", "reduce_general", "Context",
+    "ParahtError",
     "build", "column_major", "batched", "adjoint",
 ]
 
@@ -161,7 +162,7 @@ def _dispatch(host_name, dev_name, what, n, mats, ints, flags, ctx=None):
         args = [c._h, n]
         for (m, _), ld in zip(mats, lds):
             args += [ctypes.c_void_p(m.data_ptr()), ld]
-        st = getattr(L, dev_name)(*args, *ints, flags)
+        st = getattr(L, dev_name)(*args, *ints, *([] if flags is None else [flags]))
         _lib.check(st, what)
         return
     args = [n]
@@ -234,6 +235,39 @@ def reduce(A, B, r, p, nq, Q=None, Z=None, time_components=False, ctx=None):
     _dispatch("paraht_reduce", None, "reduce", n,
               [(H, n), (T, n), (Q, n), (Z, n)], [r, p, nq], flags)
     return H, T, Q, Z
+
+
+def qr_b(n, A, lda, B, ldb, Q, ldq, nb=64, p=2, ctx=None):
+    """B <- R = Q_B^T B (upper triangular), A <- Q_B^T A, Q <- Q Q_B, in
+    place, on the GPU: the step the reference's driver does with LAPACK
+    before the reduction (src/profilereduction.f90:117-123).  nb columns per
+    panel, panel blocks of p*nb rows."""
+    _dispatch("paraht_qr_b", "paraht_qr_b_dev", "qr_b", n,
+              [(A, lda), (B, ldb), (Q, ldq)], [nb, p], None, ctx)
+
+
+def reduce_general(A, B, r, p, nq, Q=None, Z=None, qr_nb=64, qr_p=2,
+                   time_components=False, ctx=None):
+    """Reduction of a pencil with a general (not triangular) B: the QR of B
+    (qr_b), then both stages.  Returns (H, T, Q, Z) with Q^T A0 Z = H and
+    Q^T B0 Z = T for the input pencil (A0, B0) and identity Q, Z; NumPy
+    inputs are copied, torch CUDA inputs (column-major) reduced in place."""
+    n = A.shape[0]
+    for M, nm in ((A, "A"), (B, "B"), (Q, "Q"), (Z, "Z")):
+        if M is not None and tuple(M.shape) != (n, n):
+            raise ValueError(f"reduce_general: {nm} has shape {tuple(M.shape)}, "
+                             f"expected ({n}, {n})")
+    if _is_torch(A):
+        import torch
+        if Q is None:
+            Q = column_major(torch.eye(n, dtype=torch.float64, device=A.device))
+        qr_b(n, A, None, B, None, Q, None, qr_nb, qr_p, ctx=ctx)
+        return reduce(A, B, r, p, nq, Q, Z, time_components, ctx)
+    H = np.array(A, dtype=np.float64, order="F", copy=True)
+    T = np.array(B, dtype=np.float64, order="F", copy=True)
+    Q = np.eye(n, order="F") if Q is None else np.array(Q, dtype=np.float64, order="F", copy=True)
+    qr_b(n, H, n, T, n, Q, n, qr_nb, qr_p)
+    return reduce(H, T, r, p, nq, Q, Z, time_components)
 
 
 from . import batched  # noqa: E402  (batched reduction of many small pencils)

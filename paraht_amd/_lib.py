@@ -46,6 +46,8 @@ SIGNATURES = {
     "paraht_rht2ht_dev": ([_P] + _SIG_HOST12, _I),
     "paraht_reduce_dev": ([_P] + _SIG_HOST12 + [_I], _I),
     "paraht_reduce_dev_noalloc": ([_P] + _SIG_HOST12 + [_I], _I),
+    "paraht_qr_b": (_SIG_HOST12[:7] + [_I, _I], _I),
+    "paraht_qr_b_dev": ([_P] + _SIG_HOST12[:7] + [_I, _I], _I),
     "paraht_batched_dev": ([_P, _I, ctypes.POINTER(Pencil), _DP, _DP, _DP, _DP, _I], _I),
     "paraht_batched_defaults": ([_I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                  ctypes.POINTER(_I)], _I),

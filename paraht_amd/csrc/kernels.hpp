@@ -61,6 +61,17 @@ __host__ __device__ inline S1Block s1_block(int n, int nb, int p, int jb0, int n
 // updates of A and Z (rows 0..n) and of B (rows 0..i0+len).  V and T of
 // block bl sit at bl*p*nb*nb and bl*nb*nb of the panel's tapes.
 hipError_t s1_block_lists(hipStream_t st, ApplyBlock* lists, int nd, int n, int nb, int p);
+// QR of a general B by panels of nb columns (the reference driver's dgeqrf,
+// src/profilereduction.f90:117): panel q covers columns
+// [qr_panel_col(q), +nb) and rows from its first column down, factored by the
+// stage-1 panel kernels on B shifted up by nb rows (order n + nb).  Three
+// block lists of nd blocks each: the left update of B's trailing columns,
+// the left update of A (all columns), the right update of Q (all rows).
+__host__ __device__ inline int qr_npanels(int n, int nb) { return (n + nb - 1) / nb; }
+__host__ __device__ inline int qr_panel_col(int n, int nb, int q) {
+  return q * nb < n - nb ? q * nb : n - nb;
+}
+hipError_t qr_block_lists(hipStream_t st, ApplyBlock* lists, int nd, int n, int nb, int p);
 
 // Bottom-up block QRs of the panel A(jb+nb:n, jb:jb+nb-1) (0-based jb);
 // writes explicit V (ldv = Pmax rows per block, vstride per block) and T.

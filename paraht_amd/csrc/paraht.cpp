@@ -521,6 +521,80 @@ int stage1(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
 }
 
 // ------------------------------------------------------------------------
+// QR of a general B (the step before stage 1)
+// ------------------------------------------------------------------------
+// The reference's driver makes B upper triangular with LAPACK dgeqrf on the
+// host and keeps R (src/profilereduction.f90:117-123).  Here B = Q_B R on the GPU by
+// panels of nb columns: each panel's rows from its first column down are
+// factored bottom-up in blocks of p*nb rows that overlap by nb rows (the
+// stage-1 panel kernels, on B shifted up by nb rows), and the panel's block
+// reflectors are applied to B's trailing columns, to A from the left and to
+// Q from the right: B <- R, A <- Q_B^T A, Q <- Q Q_B, so a pencil (A, B)
+// with any B reduces with Q^T A0 Z = H, Q^T B0 Z = T.  The last panel starts
+// at column n - nb; its columns already triangular get identity reflectors.
+size_t qr_ws_bytes(int n, int nb, int p) {
+  const int P = p * nb;
+  const int nbmax = pht::s1_nblocks(n + nb, nb, p, 0);
+  const int np = pht::qr_npanels(n, nb);
+  size_t nd = 0;
+  for (int q = 0; q < np; ++q) nd += pht::s1_nblocks(n + nb, nb, p, pht::qr_panel_col(n, nb, q));
+  const size_t vsz = (size_t)nbmax * P * nb, tsz = (size_t)nbmax * nb * nb;
+  return (vsz + tsz + (size_t)nbmax * 64) * sizeof(double) + 3 * nd * sizeof(ApplyBlock) + 256;
+}
+
+int qr_of_b(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb, double* Q, int ldq,
+            int nb, int p) {
+  if (n < 2) return 0;
+  nb = std::min(nb, n);
+  hipStream_t st = c->stream;
+  const int P = p * nb;
+  const int nbmax = pht::s1_nblocks(n + nb, nb, p, 0);
+  const int np = pht::qr_npanels(n, nb);
+  int nd = 0;
+  for (int q = 0; q < np; ++q) nd += pht::s1_nblocks(n + nb, nb, p, pht::qr_panel_col(n, nb, q));
+  if (int e = ensure_ws(c, qr_ws_bytes(n, nb, p))) return e;
+  double* V = (double*)c->ws;
+  double* T = V + (size_t)nbmax * P * nb;
+  double* taus = T + (size_t)nbmax * nb * nb;
+  ApplyBlock* dD =
+      (ApplyBlock*)(((uintptr_t)(taus + (size_t)nbmax * 64) + 255) & ~(uintptr_t)255);
+  CK(pht::qr_block_lists(st, dD, nd, n, nb, p));
+  // the panel kernels factor rows [jb + nb, n') of columns [jb, jb + nb): on
+  // B shifted up by nb rows (every row they touch is >= nb there) with
+  // n' = n + nb, those are rows [jb, n) of B
+  double* Bup = B - nb;
+  const bool fast_gen = nb <= 64 && P <= 128 && getenv("PARAHT_GENERIC_S1") == nullptr;
+  for (int q = 0, first = 0; q < np; ++q) {
+    const int jb = pht::qr_panel_col(n, nb, q);
+    const int nblocks = pht::s1_nblocks(n + nb, nb, p, jb);
+    double qf = 0.0;
+    for (int bl = 0; bl < nblocks; ++bl) {
+      const pht::S1Block b = pht::s1_block(n + nb, nb, p, jb, nblocks, bl);
+      qf += qr_flops(b.len, b.k);
+    }
+    if (fast_gen)
+      PROF(PARAHT_K_S1_PANEL_QR, qf, 0.0,
+           pht::s1_panel_qr_fast(st, Bup, ldb, n + nb, jb, nb, p, nblocks, V, P,
+                                 (long long)P * nb, T, nb, (long long)nb * nb, taus));
+    else
+      PROF(PARAHT_K_S1_PANEL_QR, qf, 0.0,
+           pht::s1_panel_qr(st, Bup, ldb, n + nb, jb, nb, p, nblocks, V, P, (long long)P * nb,
+                            T, nb, (long long)nb * nb));
+    PROF(PARAHT_K_WY_APPLY, 0.0, 0.0,
+         pht::apply_blocks(st, pht::kLeftT, B, ldb, V, P, T, nb, dD + first, nblocks, nb, P,
+                           jb + nb, n));
+    PROF(PARAHT_K_WY_APPLY, 0.0, 0.0,
+         pht::apply_blocks(st, pht::kLeftT, A, lda, V, P, T, nb, dD + nd + first, nblocks, nb,
+                           P, 0, n));
+    PROF(PARAHT_K_WY_APPLY, 0.0, 0.0,
+         pht::apply_blocks(st, pht::kRightN, Q, ldq, V, P, T, nb, dD + 2 * (size_t)nd + first,
+                           nblocks, nb, P, 0, n));
+    first += nblocks;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------
 // Stage 2 (Givens)
 // ------------------------------------------------------------------------
 // helper workgroups per stage-2 sweep column (PARAHT_S2_HELPERS overrides, 0..4);
@@ -769,7 +843,7 @@ int download(hipStream_t st, double* h, int ld, const DevMat& m, int n) {
   return 0;
 }
 
-enum Which { kStage1 = 1, kStage2 = 2, kBoth = 3 };
+enum Which { kStage1 = 1, kStage2 = 2, kBoth = 3, kQrB = 4 };
 
 int run_dev(paraht_ctx* c, Which w, int n, double* A, int lda, double* B,
             int ldb, double* Q, int ldq, double* Z, int ldz, int r, int p,
@@ -781,7 +855,8 @@ int run_dev(paraht_ctx* c, Which w, int n, double* A, int lda, double* B,
   CK(hipEventCreate(&e2));
   CK(hipEventRecord(e0, c->stream));
   int st = 0;
-  if (w & kStage1) st = stage1(c, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, p, flags);
+  if (w & kQrB) st = qr_of_b(c, n, A, lda, B, ldb, Q, ldq, r, p);
+  if (!st && (w & kStage1)) st = stage1(c, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, p, flags);
   if (!st) {
     hipError_t he = hipEventRecord(e1, c->stream);
     if (he != hipSuccess) st = hip_fail(he, "hipEventRecord", __LINE__);
@@ -813,13 +888,18 @@ int run_host(Which w, int n, double* A, int lda, double* B, int ldb, double* Q,
   if (int e = upload(c->stream, dA, A, n, lda)) return e;
   if (int e = upload(c->stream, dB, B, n, ldb)) return e;
   if (int e = upload(c->stream, dQ, Q, n, ldq)) return e;
-  if (int e = upload(c->stream, dZ, Z, n, ldz)) return e;
-  int st = run_dev(c, w, n, dA.d, n, dB.d, n, dQ.d, n, dZ.d, n, r, p, nq, flags);
+  // the QR of B has no Z: its callers pass Q in Z's place
+  const bool has_z = Z != Q;
+  if (has_z)
+    if (int e = upload(c->stream, dZ, Z, n, ldz)) return e;
+  double* zd = has_z ? dZ.d : dQ.d;
+  int st = run_dev(c, w, n, dA.d, n, dB.d, n, dQ.d, n, zd, n, r, p, nq, flags);
   if (st) return st;
   if (int e = download(c->stream, A, lda, dA, n)) return e;
   if (int e = download(c->stream, B, ldb, dB, n)) return e;
   if (int e = download(c->stream, Q, ldq, dQ, n)) return e;
-  if (int e = download(c->stream, Z, ldz, dZ, n)) return e;
+  if (has_z)
+    if (int e = download(c->stream, Z, ldz, dZ, n)) return e;
   CK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1175,6 +1255,26 @@ int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
   if (!pht::s1_supported(r, p)) return PARAHT_ERR_UNSUPPORTED;
   if (int e = s2_check(r, nq)) return e;  // before stage 1 changes anything
   return run_dev(ctx, kBoth, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, p, nq, flags);
+}
+
+int paraht_qr_b(int n, double* A, int lda, double* B, int ldb, double* Q, int ldq, int nb,
+                int p) {
+  if (int e = check_common(n, lda, ldb, ldq, ldq)) return e;
+  if (nb < 1) return -8;
+  if (p < 2) return -9;
+  if (!pht::s1_supported(std::min(nb, std::max(n, 1)), p)) return PARAHT_ERR_UNSUPPORTED;
+  // Z is not touched: Q stands in for it in the transfer
+  return run_host(kQrB, n, A, lda, B, ldb, Q, ldq, Q, ldq, nb, p, 1, 0);
+}
+
+int paraht_qr_b_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB, int ldb,
+                    double* dQ, int ldq, int nb, int p) {
+  if (!ctx) return -1;
+  if (int e = check_common(n, lda, ldb, ldq, ldq)) return e - 1;
+  if (nb < 1) return -9;
+  if (p < 2) return -10;
+  if (!pht::s1_supported(std::min(nb, std::max(n, 1)), p)) return PARAHT_ERR_UNSUPPORTED;
+  return run_dev(ctx, kQrB, n, dA, lda, dB, ldb, dQ, ldq, dQ, ldq, nb, p, 1, 0);
 }
 
 int paraht_reduce_dev_noalloc(void* stream, int n, double* dA, int lda, double* dB, int ldb,

@@ -337,6 +337,33 @@ hipError_t s1_block_lists(hipStream_t st, ApplyBlock* lists, int nd, int n, int 
   return hipGetLastError();
 }
 
+// One workgroup per QR panel of B (qr_block_lists in kernels.hpp); the block
+// rows are given in B's own coordinates (the panel kernels see them nb rows
+// lower).
+__global__ void k_qr_block_lists(ApplyBlock* __restrict__ lists, int nd, int n, int nb, int p) {
+  const int panel = blockIdx.x;
+  int first = 0;
+  for (int q = 0; q < panel; ++q) first += s1_nblocks(n + nb, nb, p, qr_panel_col(n, nb, q));
+  const int jb = qr_panel_col(n, nb, panel);
+  const int nblocks = s1_nblocks(n + nb, nb, p, jb);
+  const int P = p * nb;
+  for (int bl = threadIdx.x; bl < nblocks; bl += blockDim.x) {
+    const S1Block b = s1_block(n + nb, nb, p, jb, nblocks, bl);
+    const long long voff = (long long)bl * P * nb, toff = (long long)bl * nb * nb;
+    const int e = first + bl, i0 = b.i0 - nb;
+    lists[e] = {i0, b.len, b.k, jb + nb, n, 0, voff, toff};  // B, trailing columns
+    lists[nd + e] = {i0, b.len, b.k, 0, n, 0, voff, toff};   // A
+    lists[2 * nd + e] = {i0, b.len, b.k, 0, n, 0, voff, toff};  // Q
+  }
+}
+
+hipError_t qr_block_lists(hipStream_t st, ApplyBlock* lists, int nd, int n, int nb, int p) {
+  const int np = qr_npanels(n, nb);
+  if (np <= 0 || nd <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_qr_block_lists, dim3(np), dim3(128), 0, st, lists, nd, n, nb, p);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 static constexpr size_t kMaxLds = 160 * 1024;
 

@@ -30,6 +30,13 @@ module paraht_capi
          real(c_double) :: A(*), B(*), Q(*), Z(*)
       end function
 
+      integer(c_int) function paraht_qr_b(n, A, lda, B, ldb, Q, ldq, nb, p) &
+            bind(C, name='paraht_qr_b')
+         import :: c_int, c_double
+         integer(c_int), value :: n, lda, ldb, ldq, nb, p
+         real(c_double) :: A(*), B(*), Q(*)
+      end function
+
       integer(c_int) function paraht_version() bind(C, name='paraht_version')
          import :: c_int
       end function

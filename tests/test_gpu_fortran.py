@@ -57,3 +57,32 @@ def test_driver_paraht_on_a_pencil_file(tmp_path):
     u = 2.0 ** -53
     assert ea <= 10 * n * u and eb <= 10 * n * u, (ea, eb)
     assert O.below_band(H, 1) == 0.0 and O.below_band(T, 0) == 0.0
+
+
+def test_driver_paraht_general_b_from_a_file(tmp_path):
+    """Algorithm 1 on a pencil file with a general B: the QR of B on the GPU
+    (paraht_qr_b through the module's binding) and both stages, checked
+    against the pencil in the file."""
+    import sys
+    import os
+    import numpy as np
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle"))
+    import oracle as O
+    from paraht_amd.pencils import load_reduced, save_pencil
+    n, r, p, nq = 300, 16, 4, 8
+    print(f"fortran driver general B n={n}", flush=True)
+    rng = np.random.default_rng(21)
+    A0, B0 = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    f, g = tmp_path / "gen.pht", tmp_path / "gen_out.pht"
+    save_pencil(f, A0, B0)
+    out = subprocess.run([dropin.DRIVER, "0", "1", str(r), str(p), str(nq), f"pencil={f}",
+                          f"out={g}"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    m = LINE.search(out.stdout)
+    assert m and int(m.group(1)) == n, out.stdout
+    H, T, Q, Z = load_reduced(g)
+    ea, eb = O.backward_errors(A0, B0, H, T, Q, Z)
+    u = 2.0 ** -53
+    assert ea <= 10 * n * u and eb <= 10 * n * u, (ea, eb)
+    assert O.below_band(H, 1) == 0.0 and O.below_band(T, 0) == 0.0
+    assert O.chordal_eig_distance(A0, B0, H, T) <= 1e-6
