@@ -3,7 +3,9 @@
 // configurations (nb = 64, p = 2; nb = 16, p = 8).  Same mathematics as the
 // generic kernels in stage1.hip (DGEQR2 / DGERQ2 + DORGRQ + DGELQ2 / DLARFT
 // semantics, src/ab2rht.f90:105-116,290-311); the difference is data
-// placement and control flow.
+// placement and control flow, and the right generator takes the DGELQ2
+// reflectors of its orthonormal Y from an LU factorisation instead of a
+// second Householder loop (the same reflectors and factors, see rq_gen_body).
 //  * A workgroup of 1024 threads holds the block in registers with G = 8 or
 //    16 threads per row (column): every thread keeps 16 (8) entries, element
 //    j of a thread in group position q being index G*j + q, so the lanes that
@@ -134,8 +136,9 @@ __device__ __forceinline__ Refl refl_from_ss(double al, double ss, const double*
 // for H = H_A H_B, T_AB = -T_AA G_AB T_BB, on FP64 MFMA: first within the two
 // 32-column halves, then between them.  Entries beyond k are zero (their
 // Gram rows and tau are masked).
-__device__ inline void gram_and_t(const double* Vs, int k, const double* tau, double* G,
-                           double* Tl, double* Tout, int ldt, int kfull, int ldvs = kLd) {
+__device__ __forceinline__ void gram_and_t(const double* Vs, int k, const double* tau,
+                                           double* G, double* Tl, double* Tout, int ldt,
+                                           int kfull, int ldvs = kLd) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int l16 = lane & 15, q = lane >> 4;
   {
@@ -143,12 +146,27 @@ __device__ inline void gram_and_t(const double* Vs, int k, const double* tau, do
     const int I = wave >> 2, J = wave & 3;
     if (I <= J) {
       const bool oka = 16 * I + l16 < k, okb = 16 * J + l16 < k;
-      const double* va = Vs + (16 * I + (oka ? l16 : 0)) * ldvs;
-      const double* vb = Vs + (16 * J + (okb ? l16 : 0)) * ldvs;
-      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-      for (int k0 = 0; k0 < 128; k0 += 4)
-        acc = mfma16(oka ? va[k0 + q] : 0.0, okb ? vb[k0 + q] : 0.0, acc);
+      const double* va = Vs + (16 * I + (oka ? l16 : 0)) * ldvs + q;
+      const double* vb = Vs + (16 * J + (okb ? l16 : 0)) * ldvs + q;
+      // the operands of eight MFMAs are loaded at once (rows past k are
+      // masked after the load), and two accumulators halve the chain
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 128; k0 += 32) {
+        double x[8], y[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          x[u] = va[k0 + 4 * u];
+          y[u] = vb[k0 + 4 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          acc = mfma16(oka ? x[u] : 0.0, okb ? y[u] : 0.0, acc);
+          acc1 = mfma16(oka ? x[u + 1] : 0.0, okb ? y[u + 1] : 0.0, acc1);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] += acc1[i];
 #pragma unroll
       for (int i = 0; i < 4; ++i) G[(16 * I + 4 * i + q) * 65 + 16 * J + l16] = acc[i];
     }
@@ -490,13 +508,22 @@ __device__ inline void rq_gen_body(
       const int b = 16 * wave;
       double* slot = Tb + wave * 272;
       const bool ok = b + l16 < len;
-      const double* va = Vr + (ok ? b + l16 : 0) * kLd;
-      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-      for (int k0 = 0; k0 < 128; k0 += 4) {
-        const double x = ok ? va[k0 + qq] : 0.0;
-        acc = mfma16(x, x, acc);
+      const double* va = Vr + (ok ? b + l16 : 0) * kLd + qq;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 128; k0 += 32) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = va[k0 + 4 * u];
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          const double x0 = ok ? x[u] : 0.0, x1 = ok ? x[u + 1] : 0.0;
+          acc = mfma16(x0, x0, acc);
+          acc1 = mfma16(x1, x1, acc1);
+        }
       }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] += acc1[i];
 #pragma unroll
       for (int i = 0; i < 4; ++i) slot[(4 * i + qq) * 17 + l16] = acc[i];
       __builtin_amdgcn_wave_barrier();
@@ -537,16 +564,23 @@ __device__ inline void rq_gen_body(
         const double* slot = Tb + m * 272;
         const bool vok = b + l16 < len;
         const double* va = Vr + (vok ? b + l16 : 0) * kLd;
-        dbl4 wt = {0.0, 0.0, 0.0, 0.0};
+        dbl4 wt = {0.0, 0.0, 0.0, 0.0}, wt1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int ct = 0; ct < 8; ++ct) {
           if (ct > m) continue;
           double av[4];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) av[i] = vok ? va[16 * ct + 4 * i + qq] : 0.0;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) wt = mfma16(av[i], Yt[ct][i], wt);
+          for (int i = 0; i < 4; ++i) {
+            const double x = va[16 * ct + 4 * i + qq];
+            av[i] = vok ? x : 0.0;
+          }
+          wt = mfma16(av[0], Yt[ct][0], wt);
+          wt1 = mfma16(av[1], Yt[ct][1], wt1);
+          wt = mfma16(av[2], Yt[ct][2], wt);
+          wt1 = mfma16(av[3], Yt[ct][3], wt1);
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) wt[i] += wt1[i];
         double tv[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) tv[i] = slot[(4 * i + qq) * 17 + l16];
@@ -559,8 +593,9 @@ __device__ inline void rq_gen_body(
           double av[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int kk = 4 * i + qq;
-            av[i] = (b + kk < len) ? -Vr[(b + kk) * kLd + 16 * ct + l16] : 0.0;
+            const int kk = 4 * i + qq;  // row b + kk < 128: in the buffer
+            const double x = Vr[(b + kk) * kLd + 16 * ct + l16];
+            av[i] = (b + kk < len) ? -x : 0.0;
           }
 #pragma unroll
           for (int i = 0; i < 4; ++i) Yt[ct] = mfma16(av[i], w2[i], Yt[ct]);
@@ -577,66 +612,73 @@ __device__ inline void rq_gen_body(
     }
     __syncthreads();
   }
-  constexpr int NJ = 8;
-  const int a = tid >> 4, q = tid & 15;
-  double Y[NJ];
-#pragma unroll
-  for (int j = 0; j < NJ; ++j) Y[j] = (a < nb) ? Ys[a * kLd + 16 * j + q] : 0.0;
-
-  // ---- DGELQ2 on Y (nb x len): reflector i = 16 jj + qp from row i, zero
-  // left of column i, applied to the rows below it; only chunks j >= jj take
-  // part.  Broadcast rows in Tb (free again after the Y phase).
-  double mysc = 1.0;
+  // ---- The reflectors of DGELQ2 on Y (nb x len, orthonormal rows), from
+  // the LU factorisation without pivoting of W = [S; 0] - Y^T, S diagonal
+  // +-1 (Householder reconstruction, Ballard, Demmel, Grigori, Jacquelin,
+  // Nguyen and Solomonik, "Reconstructing Householder vectors from
+  // tall-skinny QR", IPDPS 2014): S_jj is the sign of W's current diagonal
+  // entry, so every pivot has magnitude 1 + |W_jj| >= 1; then V is L (unit
+  // lower trapezoidal) and tau_j = S_jj U_jj, which are DLARFG's reflectors
+  // and factors for the same signs.  Step j needs no reduction: the owners of
+  // W's column j and row j publish them in LDS (two alternating slots), and
+  // after one barrier every lane forms its multiplier and updates its
+  // entries.  Lane layout: W row i (a column of Y) on 8 lanes, entry k = 8m+h.
   {
-    double* bc = Tb;
+    constexpr int NM = 8;
+    const int i = tid >> 3, h = tid & 7;
+    const bool live = i < len;
+    double Wr[NM];
 #pragma unroll
-    for (int jj = 0; jj < NJ; ++jj) {
-      for (int qp = 0; qp < 16; ++qp) {
-        const int i = 16 * jj + qp;
-        if (i >= nb) break;
-        double* vb = bc + (i & 1) * 128;
-        if (a == i) {
-          const double al = grp_sum<16>(q == qp ? Y[jj] : 0.0);
-          double s0 = (q > qp) ? Y[jj] * Y[jj] : 0.0, s1 = 0.0;
+    for (int m = 0; m < NM; ++m) {
+      const int k = 8 * m + h;
+      Wr[m] = (live && k < nb) ? -Ys[k * kLd + i] : 0.0;
+    }
+    double* colb = Tb;        // [2][128]: W(:, j)
+    double* rowb = Tb + 256;  // [2][64]: W(j, :)
 #pragma unroll
-          for (int j = jj + 1; j < NJ; j += 2) {
-            s0 += Y[j] * Y[j];
-            if (j + 1 < NJ) s1 += Y[j + 1] * Y[j + 1];
-          }
-          const double ss = grp_sum<16>(s0 + s1);
-          const Refl R = refl_from_ss<16, NJ>(al, ss, Y, [&](int j) { return 16 * j + q > i; });
-          const double sc = (R.tau != 0.0) ? R.scal : 1.0;
-          mysc = sc;
-          vb[16 * jj + q] = (q > qp) ? Y[jj] * sc : (q == qp ? 1.0 : 0.0);
+    for (int m = 0; m < NM; ++m) {
+      if (8 * m >= nb) break;
+      for (int hp = 0; hp < 8; ++hp) {
+        const int j = 8 * m + hp;
+        if (j >= nb) break;
+        double* cb = colb + (j & 1) * 128;
+        double* rb = rowb + (j & 1) * 64;
+        if (h == hp && live && i >= j) cb[i] = Wr[m];
+        if (i == j) {
 #pragma unroll
-          for (int j = jj + 1; j < NJ; ++j) vb[16 * j + q] = Y[j] * sc;
-          if (q == 0) tau2[i] = R.tau;
+          for (int mm = m; mm < NM; ++mm) rb[8 * mm + h] = Wr[mm];
         }
         __syncthreads();
-        const double t = tau2[i];
-        if (a > i && a < nb && t != 0.0) {
-          double v[NJ];
+        // every read of the step is issued at once (all indices are in range)
+        const double c = cb[j], ci = cb[i];
+        double rv[NM];
 #pragma unroll
-          for (int j = jj; j < NJ; ++j) v[j] = vb[16 * j + q];
-          double w0 = 0.0, w1 = 0.0;
+        for (int mm = m; mm < NM; ++mm) rv[mm] = rb[8 * mm + h];
+        const double sj = (c >= 0.0) ? 1.0 : -1.0;
+        const double pv = c + sj;  // |pv| in [1, 2]: v_rcp_f64 and two Newton steps
+        double inv = __builtin_amdgcn_rcp(pv);
+        double e = fma(-pv, inv, 1.0);
+        inv = fma(inv, e, inv);
+        e = fma(-pv, inv, 1.0);
+        inv = fma(inv, e, inv);
+        if (live && i > j) {
+          const double l = ci * inv;
 #pragma unroll
-          for (int j = jj; j < NJ; j += 2) {
-            w0 += Y[j] * v[j];
-            if (j + 1 < NJ) w1 += Y[j + 1] * v[j + 1];
+          for (int mm = m; mm < NM; ++mm) {
+            const int k = 8 * mm + h;
+            Wr[mm] = (k > j && k < nb) ? fma(-l, rv[mm], Wr[mm]) : Wr[mm];
           }
-          const double w = grp_sum<16>(w0 + w1) * t;
-#pragma unroll
-          for (int j = jj; j < NJ; ++j) Y[j] -= w * v[j];
+          if (h == hp) Wr[m] = l;
         }
+        if (i == j && h == hp) tau2[j] = pv * sj;
       }
     }
-  }
-  // reflector i in row i of Ys: zero left of column i, 1 on it
-  if (a < nb) {
+    __syncthreads();  // the slots are read; Ys is free
+    // reflector k in row k of Ys: zero left of column k, 1 on it
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-      const int c = 16 * j + q;
-      Ys[a * kLd + c] = (c > a) ? Y[j] * mysc : (c == a ? 1.0 : 0.0);
+    for (int m = 0; m < NM; ++m) {
+      const int k = 8 * m + h;
+      if (k < nb) Ys[k * kLd + i] = (live && i > k) ? Wr[m] : (i == k ? 1.0 : 0.0);
     }
   }
   __syncthreads();
@@ -654,6 +696,7 @@ constexpr size_t kRqLds = (size_t)(128 * kLd + 128 + 64 + 8 * 272) * sizeof(doub
 static_assert(kPanelLds <= 160 * 1024 && kRqLds <= 160 * 1024, "LDS budget");
 static_assert(kPanelLds <= kRqLds, "the batched kernel sizes its LDS for both bodies by kRqLds");
 static_assert(8 * 272 >= 256, "two broadcast rows fit in the Y phase's T slots");
+static_assert(8 * 272 >= 2 * 128 + 2 * 64, "the LU's column and row slots fit there too");
 static_assert(64 * kLd + 2 * 64 * 65 <= 128 * kLd + 128, "Gram/T alias in the RQ kernel");
 
 }  // namespace s1
