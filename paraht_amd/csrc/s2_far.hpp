@@ -376,6 +376,13 @@ __device__ __forceinline__ void wf_steps(double (&w)[R + NQ - 1], double* col,
   if constexpr (TAU + 1 < R + NQ - 2) wf_coeffs<R, NQ, TAU + 1, 0>(nx, tb);
   wf_rotations<R, NQ, TAU, 0>(w, cs, icm0, icm1);
   wf_store<R, NQ, LDT, TAU, 0>(w, col);
+  // In the split form another lane of the wave reads what these stores
+  // write, at addresses this thread never reads: seen from one thread the
+  // loads below are independent of the stores, and the compiler may issue
+  // them first (it did at four lanes and nq = 4, merging a step's loads into
+  // the previous step's).  The barrier keeps program order, which the LDS
+  // then keeps within the wave.
+  asm volatile("" ::: "memory");
   wf_load<R, NQ, LDT, TAU, 0>(w, col);
   // one step's coefficients and streamed entries in flight at a time
   asm volatile("" ::: "memory");
