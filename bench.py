@@ -33,10 +33,18 @@ KERNEL_CLASSES = {
     "wy_apply": ("k_apply_blocks_lds (compact-WY block reflectors, FP64 MFMA)", "mfma"),
     "s1_panel_qr": ("k_s1_panel_qr_tau + k_s1_panel_t (panel QR chain + T)", "latency"),
     "s1_rq_gen": ("k_s1_rq_fast (RQ + LQ right generator + T)", "latency"),
+    "s2_gen": ("k_hh_sweep (pipelined Householder bulge chase of one sweep)", "latency"),
+    "s2_rows": ("k_hh_far (compact-WY reflectors on rows of Q, Z and of A, B above the "
+                "window, FP64 MFMA)", "hbm"),
+    "s2_cols": ("k_hh_far<cols> (compact-WY reflectors on columns of A, B right of the "
+                "window, FP64 MFMA)", "hbm"),
+}
+# the same classes for the Givens stage 2 (rht2ht_givens)
+KERNEL_CLASSES_GIVENS = dict(KERNEL_CLASSES, **{
     "s2_gen": ("k_s2_sweep (pipelined Givens bulge chase of one sweep)", "latency"),
     "s2_rows": ("k_s2_rows_w2 (rotations on rows of Q, Z and of A, B above the window)", "hbm"),
     "s2_cols": ("k_s2_cols_w2 (rotations on columns of A, B right of the window)", "hbm"),
-}
+})
 
 
 def chase_steps(n, r, nq):
@@ -261,6 +269,9 @@ def main():
                     help="blocks per stage-1 reflector block (0: the tuned table, else 2)")
     ap.add_argument("--nq", type=int, default=0,
                     help="columns per stage-2 sweep (0: the tuned table, else 16)")
+    ap.add_argument("--stage2", choices=("householder", "givens"), default="householder",
+                    help="stage-2 variant: rht2ht_householder (the reference driver's) or "
+                         "rht2ht_givens")
     ap.add_argument("--cpu-n", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -286,6 +297,8 @@ def main():
     if not args.nq:
         args.nq = tuned[1] if tuned else 16
     n, r, p, nq = args.n, args.r, args.p, args.nq
+    givens = args.stage2 == "givens"
+    classes = KERNEL_CLASSES_GIVENS if givens else KERNEL_CLASSES
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
     g.manual_seed(1302)
@@ -305,7 +318,7 @@ def main():
         Q.copy_(eye)
         Z.copy_(eye)
         torch.cuda.synchronize()
-        ph.reduce(A, B, r, p, nq, Q, Z, ctx=ctx)
+        ph.reduce(A, B, r, p, nq, Q, Z, ctx=ctx, givens=givens)
 
     for i in range(args.warmup):
         step()
@@ -354,7 +367,7 @@ def main():
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"one FP64 pencil n={n}, r={r}, p={p}, nq={nq}; "
                                "A~N(0,1), B=R of QR of N(0,1), Q=Z=I",
-                   "n": n, "r": r, "p": p, "nq": nq, "stage2": "givens",
+                   "n": n, "r": r, "p": p, "nq": nq, "stage2": args.stage2,
                    "p_nq_from": "tuned table paraht_amd/tuned.json" if tuned else "fixed"},
         "gflops": flops / t_red / 1e9,
         "pct_fp64_peak": 100.0 * flops / t_red / FP64_PEAK,
@@ -374,16 +387,16 @@ def main():
             ms_g, cnt_g = kstats["s2_gen"][0], kstats["s2_gen"][1]
             steps = chase_steps(n, r, nq)
             out["latency_bound"] = {
-                "kernel": KERNEL_CLASSES["s2_gen"][0],
+                "kernel": classes["s2_gen"][0],
                 "launches": cnt_g // prof_steps, "avg_launch_us": 1e3 * ms_g / cnt_g,
                 "chase_steps": steps, "us_per_step": 1e3 * ms_g / prof_steps / steps}
         # the roofline entry: the throughput kernel class with the most time
-        thr = [k for k in kstats if KERNEL_CLASSES.get(k, ("", ""))[1] in ("mfma", "hbm")
+        thr = [k for k in kstats if classes.get(k, ("", ""))[1] in ("mfma", "hbm")
                and kstats[k][1]]
         if thr:
             dom = max(thr, key=lambda k: kstats[k][0])
             ms_k, cnt, fl, by = kstats[dom]
-            name, bound = KERNEL_CLASSES[dom]
+            name, bound = classes[dom]
             t_launch = ms_k / cnt / 1e3  # average launch duration (HIP events), s
             if bound == "mfma":
                 ach = fl / cnt / t_launch / 1e12

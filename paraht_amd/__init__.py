@@ -208,12 +208,16 @@ def rht2ht_householder_unblocked(n, A, lda, B, ldb, Q, ldq, Z, ldz, r):
               n, [(A, lda), (B, ldb), (Q, ldq), (Z, ldz)], [r], None)
 
 
-def reduce(A, B, r, p, nq, Q=None, Z=None, time_components=False, ctx=None):
+def reduce(A, B, r, p, nq, Q=None, Z=None, time_components=False, ctx=None, givens=False):
     """Both stages (src/profilereduction.f90:136,144).  Returns (H, T, Q, Z).
 
-    NumPy inputs are not modified (copies are reduced); torch CUDA inputs
-    (column-major) are reduced in place and returned."""
+    Stage 2 is rht2ht_householder, the reference driver's variant, or with
+    givens=True rht2ht_givens.  NumPy inputs are not modified (copies are
+    reduced); torch CUDA inputs (column-major) are reduced in place and
+    returned."""
     flags = PARAHT_TIME_COMPONENTS if time_components else 0
+    if givens:
+        flags |= PARAHT_STAGE2_GIVENS
     n = A.shape[0]
     for M, nm in ((A, "A"), (B, "B"), (Q, "Q"), (Z, "Z")):
         if M is not None and tuple(M.shape) != (n, n):
@@ -247,7 +251,7 @@ def qr_b(n, A, lda, B, ldb, Q, ldq, nb=64, p=2, ctx=None):
 
 
 def reduce_general(A, B, r, p, nq, Q=None, Z=None, qr_nb=64, qr_p=2,
-                   time_components=False, ctx=None):
+                   time_components=False, ctx=None, givens=False):
     """Reduction of a pencil with a general (not triangular) B: the QR of B
     (qr_b), then both stages.  Returns (H, T, Q, Z) with Q^T A0 Z = H and
     Q^T B0 Z = T for the input pencil (A0, B0) and identity Q, Z; NumPy
@@ -262,12 +266,12 @@ def reduce_general(A, B, r, p, nq, Q=None, Z=None, qr_nb=64, qr_p=2,
         if Q is None:
             Q = column_major(torch.eye(n, dtype=torch.float64, device=A.device))
         qr_b(n, A, None, B, None, Q, None, qr_nb, qr_p, ctx=ctx)
-        return reduce(A, B, r, p, nq, Q, Z, time_components, ctx)
+        return reduce(A, B, r, p, nq, Q, Z, time_components, ctx, givens)
     H = np.array(A, dtype=np.float64, order="F", copy=True)
     T = np.array(B, dtype=np.float64, order="F", copy=True)
     Q = np.eye(n, order="F") if Q is None else np.array(Q, dtype=np.float64, order="F", copy=True)
     qr_b(n, H, n, T, n, Q, n, qr_nb, qr_p)
-    return reduce(H, T, r, p, nq, Q, Z, time_components)
+    return reduce(H, T, r, p, nq, Q, Z, time_components, givens=givens)
 
 
 from . import batched  # noqa: E402  (batched reduction of many small pencils)

@@ -177,6 +177,57 @@ hipError_t s2_far_rows(hipStream_t st, const S2FarArgs& a, bool ab, const S2Rang
 template <int R, int NQ>
 hipError_t s2_far_cols(hipStream_t st, const S2FarArgs& a, const S2Range& rg);
 
+// ---- stage 2, Householder variant (rht2ht_householder semantics) ----
+// Tape of one sweep, doubles from `base`: U, V ((r+nq) x nq x nblocks each),
+// tauU, tauV (nq x nblocks each) -- zeroed by hh_prepare -- then TU, TV
+// (nq x nq x nblocks each).
+struct HhTape {
+  double *U, *V, *tauU, *tauV, *TU, *TV;
+};
+inline size_t hh_tape_doubles(int r, int nq, int nbl) {
+  return (size_t)2 * (r + nq) * nq * nbl + (size_t)2 * nq * nbl + (size_t)2 * nq * nq * nbl;
+}
+inline HhTape hh_tape(double* base, int r, int nq, int nbl) {
+  HhTape t;
+  const size_t u = (size_t)(r + nq) * nq * nbl, q = (size_t)nq * nbl;
+  t.U = base;
+  t.V = t.U + u;
+  t.tauU = t.V + u;
+  t.tauV = t.tauU + q;
+  t.TU = t.tauV + q;
+  t.TV = t.TU + (size_t)nq * nq * nbl;
+  return t;
+}
+// (r, nq) the Householder kernels take (r <= 64, nq <= 32, r + nq <= 96).
+bool hh_supported(int r, int nq);
+// (r, nq) with MFMA far-update kernels; the others use a generic kernel.
+bool hh_far_supported(int r, int nq);
+// Dynamic LDS of the sweep kernel.
+size_t hh_sweep_lds(int r);
+// Zeroes nd doubles of the tape (U, V, tauU, tauV) and nc counters; the
+// call's first sweep clears *errflag.
+hipError_t hh_prepare(hipStream_t st, double* tape, size_t nd, int* counters, int nc,
+                      int* errflag, int clear_err);
+// Generation of the sweep at column s (1-based): nq leader workgroups, each
+// with nhelp helper workgroups (0: the leaders do the rows above the window).
+hipError_t hh_generate(hipStream_t st, double* A, int lda, double* B, int ldb, int n, int r,
+                       int nq, int s, double* U, double* V, double* tauU, double* tauV,
+                       int* progress, int* hprog, int* errflag, int spin, int nhelp);
+// T factors of every block of the sweep (DLARFT 'F','C').
+hipError_t hh_factors(hipStream_t st, const double* U, const double* V, const double* tauU,
+                      const double* tauV, double* TU, double* TV, int ldu, int nq, int nblocks);
+// Phases of the far updates: Q and Z (all rows, all reflectors); A/B rows or
+// columns in one pass; the lookahead part; the rest.
+enum HhPhase { kHhFull = 0, kHhAll = 1, kHhLook = 2, kHhFar = 3 };
+// Far updates of one or two matrices (M1 = nullptr: one) of the sweep at s:
+// cols = false: rows (right reflectors Y with factor T), cols = true: columns
+// right of the window (left reflectors; isA selects A's stop columns).
+// generic forces the non-MFMA kernel.
+hipError_t hh_far(hipStream_t st, bool cols, double* M0, int ld0, double* M1, int ld1,
+                  const double* Y0, const double* T0, const double* tau0, const double* Y1,
+                  const double* T1, const double* tau1, int isA0, int isA1, int n, int r, int nq,
+                  int s, int nblocks, int phase, bool generic = false);
+
 // ---- batched reduction of many small pencils ----
 // One pencil of a batch: its n x n column-major matrices (ld = n) start at
 // element `offset` of each of the A, B, Q, Z pools.

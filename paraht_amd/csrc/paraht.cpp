@@ -803,6 +803,222 @@ int stage2(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb,
 }
 
 // ------------------------------------------------------------------------
+// Stage 2 (Householder, rht2ht_householder, src/rht2ht.f90:1346-2051)
+// ------------------------------------------------------------------------
+size_t s2h_ws_bytes(int n, int r, int nq, int nbuf) {
+  if (n <= 2) return 0;
+  const int nbl = (n - 1) / r + ((n - 1) % r != 0);
+  return (size_t)nbuf * pht::hh_tape_doubles(r, nq, nbl) * sizeof(double) +
+         pht::s2_counter_ints(nq) * sizeof(int);
+}
+
+// algorithmic work of a sweep's Householder generation: per step the left
+// reflector on the B block (4 nb^2), the block's RQ (4/3 nb^3) and the first
+// row of its orthogonal factor (2 nb^2), the right reflector on the A window,
+// the B block and the rows above (4 nb (rows)); the deferred reflectors
+// (4 r per earlier column, on two vectors)
+double s2h_gen_flops(int n, int r, int nq, int s) {
+  double f = 0.0;
+  for (int ic = 1; ic <= nq; ++ic) {
+    const int jc = s + ic - 1;
+    for (int ib = 1;; ++ib) {
+      if (ib == 1 ? jc > n : jc + 1 + (ib - 2) * r > n) break;
+      const int row0 = s + (ib - 1) * r + ic;
+      const int nb = std::min(r, n - row0 + 1);
+      f += 8.0 * r * (ic - 1);
+      if (nb < 2) continue;
+      const int istart = ib == 1 ? s + 1 : std::max(s + 1, s + 1 + (ib - 1 - nq + ic) * r);
+      const int rows = std::min(n, row0 + r + nb - 1) - row0 + 1 + nb + 2 * (row0 - istart);
+      f += 6.0 * nb * nb + (4.0 / 3.0) * nb * nb * nb + 4.0 * nb * rows;
+    }
+  }
+  return f;
+}
+
+// algorithmic bytes of a far-update launch: every entry of the windows it
+// updates read and written once per block (the rows or columns in the
+// phase's range; 16 bytes per entry and block)
+double s2h_far_bytes(int n, int r, int nq, int s, int nblocks, bool cols, int phase, int nmat) {
+  double b = 0.0;
+  const int W = r + nq;
+  for (int ib = 1; ib <= nblocks; ++ib) {
+    const int w0 = s + (ib - 1) * r;
+    const int w = std::min(W, n - w0);
+    if (w <= 0) continue;
+    long long cnt = 0;
+    if (phase == pht::kHhFull) {
+      cnt = n;
+    } else if (cols) {
+      const int istop = std::min(n, s + (ib - 1) * r + r + nq - 1);  // B's
+      const int istop2 = std::min(n, s + (ib - 1) * r + r + nq + r * nq - 1);
+      cnt = phase == pht::kHhLook ? istop2 - istop : (phase == pht::kHhFar ? n - istop2 : n - istop);
+    } else {
+      const int base = s + 1 + (ib - 1) * r;
+      const int top = std::min(n, base - 1);
+      const int look_lo = std::max(s, base - 1 - 2 * nq * r);
+      cnt = phase == pht::kHhLook ? top - look_lo : (phase == pht::kHhFar ? look_lo : top);
+    }
+    b += 16.0 * w * std::max(0LL, cnt);
+  }
+  return b * nmat;
+}
+
+// far-update flops of a launch: 4 W nq (+ 2 nq^2) per row or column and block
+double s2h_far_flops(double bytes, int r, int nq) {
+  const double W = r + nq;
+  return bytes / (16.0 * W) * (4.0 * W * nq + 2.0 * nq * nq);
+}
+
+// The generic (non-MFMA) far-update kernels for every (r, nq): PARAHT_HH_GENERIC=1
+bool s2h_generic() {
+  const char* e = getenv("PARAHT_HH_GENERIC");
+  return e && *e == '1';
+}
+
+// Per sweep: the generation (leaders + helpers), the blocks' T factors, then
+//  * Q and Z on the third stream beside everything (rht2ht.f90:1976-2018);
+//  * the A/B lookahead on the call's stream: rows within 2 nq blocks above
+//    each window, then columns within r nq of it (rht2ht.f90:1644-1765);
+//  * the rest of A and B on the second stream beside the next sweep's
+//    generation (rht2ht.f90:1818-1951), which touches none of its entries;
+//    the next lookahead waits for it.
+// A single-stream context runs the row and column updates of A and B in one
+// pass each.
+int stage2_hh(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb, double* Q, int ldq,
+              double* Z, int ldz, int r, int nq, int flags) {
+  if (n <= 2) return 0;  // rht2ht.f90:1417 loop is empty
+  if (!pht::hh_supported(r, nq)) return PARAHT_ERR_UNSUPPORTED;
+  hipStream_t st = c->stream;
+  const int ldu = r + nq;
+  const int nbl = (n - 1) / r + ((n - 1) % r != 0);
+  const int nbuf = c->single ? 1 : 2;
+  if (int e = ensure_ws(c, s2h_ws_bytes(n, r, nq, nbuf))) return e;
+  if (int e = ensure_aux(c)) return e;
+  const size_t td = pht::hh_tape_doubles(r, nq, nbl);
+  const pht::HhTape tape[2] = {pht::hh_tape((double*)c->ws, r, nq, nbl),
+                               pht::hh_tape((double*)c->ws + (nbuf - 1) * td, r, nq, nbl)};
+  int* counters = (int*)((double*)c->ws + nbuf * td);
+  int* progress = counters;
+  int* hprog = counters + nq;
+  int* errflag = counters + (size_t)nq * (1 + pht::kS2MaxHelpers);
+  const size_t zero_d = (size_t)2 * ldu * nq * nbl + (size_t)2 * nq * nbl;  // U, V, tauU, tauV
+  hipStream_t sq = side2(c);
+  const bool tc = (flags & PARAHT_TIME_COMPONENTS) != 0;
+  // a single-stream context keeps the split order on its one stream, so its
+  // results are bit-identical to the two-stream schedule's
+  const int look = c->single ? 2 : s2_lookahead();
+  const bool split = look != 0;
+  hipStream_t rest = look == 2 ? st : c->aux;
+  const bool gen_far = s2h_generic();
+  Timer tm(c, tc);
+  std::vector<int> marks;
+  int buf = 0;
+  bool far_pending = false;
+  // the Q/Z stream starts after everything on the call's stream so far
+  if (int e = rec(c, c->ev_tape, st)) return e;
+  if (int e = wait_on(c, sq, c->ev_tape)) return e;
+  for (int s = 1, k = 0; s <= n - 2; s += nq, ++k) {
+    buf = k & 1;
+    const pht::HhTape& T = tape[buf];
+    const int nblocks = (n - s) / r + ((n - s) % r != 0);
+    // helpers need every workgroup resident and nq <= r+1 (a column's steps
+    // then never read the rows above its earlier bulges)
+    int nhelp = (nq <= r + 1) ? std::min(s2_helpers(), c->cu_count / nq - 1) : 0;
+    if (nhelp < 0) nhelp = 0;
+    const int spin = (k == 0) ? s2_first_spin() : pht::kS2SpinLimit;
+    const int m0 = tm.mark();
+    // the buffer's previous tape (sweep k-2) must be consumed by its Q/Z update
+    // (its A/B rest finished before sweep k-1's lookahead)
+    if (k >= 2)
+      if (int e = wait_on(c, st, c->ev_free[buf])) return e;
+    PROF(PARAHT_K_OTHER, 0.0, 0.0,
+         pht::hh_prepare(st, T.U, zero_d, counters, (int)((size_t)nq * (1 + pht::kS2MaxHelpers)),
+                         errflag, k == 0));
+    PROF(PARAHT_K_S2_GEN, s2h_gen_flops(n, r, nq, s), 0.0,
+         pht::hh_generate(st, A, lda, B, ldb, n, r, nq, s, T.U, T.V, T.tauU, T.tauV, progress,
+                          hprog, errflag, spin, nhelp));
+    PROF(PARAHT_K_OTHER, 0.0, 0.0,
+         pht::hh_factors(st, T.U, T.V, T.tauU, T.tauV, T.TU, T.TV, ldu, nq, nblocks));
+    const int m1 = tm.mark();
+    if (int e = rec(c, c->ev_tape, st)) return e;
+    if (int e = wait_on(c, sq, c->ev_tape)) return e;
+    {
+      const double by = s2h_far_bytes(n, r, nq, s, nblocks, false, pht::kHhFull, 2);
+      PROF_ON(sq, PARAHT_K_S2_ROWS, s2h_far_flops(by, r, nq), by,
+              pht::hh_far(sq, false, Q, ldq, Z, ldz, T.U, T.TU, T.tauU, T.V, T.TV, T.tauV, 0, 0,
+                          n, r, nq, s, nblocks, pht::kHhFull, gen_far));
+    }
+    if (int e = rec(c, c->ev_free[buf], sq)) return e;
+    auto ab = [&](hipStream_t strm, bool cols, int phase) -> int {
+      const double by = s2h_far_bytes(n, r, nq, s, nblocks, cols, phase, 2);
+      PROF_ON(strm, cols ? PARAHT_K_S2_COLS : PARAHT_K_S2_ROWS, s2h_far_flops(by, r, nq), by,
+              cols ? pht::hh_far(strm, true, A, lda, B, ldb, T.U, T.TU, T.tauU, T.U, T.TU,
+                                 T.tauU, 1, 0, n, r, nq, s, nblocks, phase, gen_far)
+                   : pht::hh_far(strm, false, A, lda, B, ldb, T.V, T.TV, T.tauV, T.V, T.TV,
+                                 T.tauV, 0, 0, n, r, nq, s, nblocks, phase, gen_far));
+      return 0;
+    };
+    if (split) {
+      // the previous sweep's rest must be done before this lookahead
+      if (far_pending)
+        if (int e = wait_on(c, st, c->ev_far)) return e;
+      if (int e = ab(st, false, pht::kHhLook)) return e;
+      if (int e = ab(st, true, pht::kHhLook)) return e;
+      if (int e = rec(c, c->ev_near, st)) return e;
+      if (int e = wait_on(c, rest, c->ev_near)) return e;
+      if (int e = ab(rest, false, pht::kHhFar)) return e;
+      if (int e = ab(rest, true, pht::kHhFar)) return e;
+      if (int e = rec(c, c->ev_far, rest)) return e;
+      far_pending = true;
+    } else {
+      if (int e = ab(st, false, pht::kHhAll)) return e;
+      if (int e = ab(st, true, pht::kHhAll)) return e;
+    }
+    const int m2 = tm.mark();
+    if (tc) marks.insert(marks.end(), {m0, m1, m2});
+  }
+  // join: the last A/B rest and the last Q/Z update
+  if (far_pending)
+    if (int e = wait_on(c, st, c->ev_far)) return e;
+  if (int e = wait_on(c, st, c->ev_free[buf])) return e;
+  {
+    int err = 0;
+    CK(hipMemcpyAsync(&err, errflag, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (err) {
+      if (verbose()) fprintf(stderr, "paraht: stage-2 sweep wait exceeded its spin limit\n");
+      return PARAHT_ERR_HIP;
+    }
+  }
+  if (tc) {
+    CK(hipStreamSynchronize(st));
+    double gen = 0, app = 0;
+    for (size_t i = 0; i < marks.size(); i += 3) {
+      gen += tm.ms(marks[i], marks[i + 1]);
+      app += tm.ms(marks[i + 1], marks[i + 2]);
+    }
+    c->times[4] = gen;
+    // the reference's line (rht2ht.f90:2045-2046, seconds): the lookahead is
+    // the call stream's share of the A/B updates
+    printf("      gen: %16.6f ;lookahead: %16.6f ;application: %16.6f\n", gen * 1e-3,
+           split ? app * 1e-3 : 0.0, split ? 0.0 : app * 1e-3);
+  }
+  return 0;
+}
+
+// The stage-2 variant of a call: PARAHT_STAGE2_GIVENS selects rht2ht_givens,
+// otherwise rht2ht_householder.
+int s2_check_variant(int r, int nq, int flags) {
+  if (flags & PARAHT_STAGE2_GIVENS) return s2_check(r, nq);
+  return pht::hh_supported(r, nq) ? 0 : PARAHT_ERR_UNSUPPORTED;
+}
+int stage2_any(paraht_ctx* c, int n, double* A, int lda, double* B, int ldb, double* Q, int ldq,
+               double* Z, int ldz, int r, int nq, int flags) {
+  if (flags & PARAHT_STAGE2_GIVENS) return stage2(c, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, nq, flags);
+  return stage2_hh(c, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, nq, flags);
+}
+
+// ------------------------------------------------------------------------
 std::mutex g_mu;
 paraht_ctx* g_default[64] = {nullptr};
 
@@ -862,7 +1078,7 @@ int run_dev(paraht_ctx* c, Which w, int n, double* A, int lda, double* B,
     if (he != hipSuccess) st = hip_fail(he, "hipEventRecord", __LINE__);
   }
   if (!st && (w & kStage2))
-    st = stage2(c, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, nq, flags);
+    st = stage2_any(c, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, nq, flags);
   if (!st) {
     hipError_t he = hipEventRecord(e2, c->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
@@ -1198,7 +1414,7 @@ int paraht_rht2ht(int n, double* A, int lda, double* B, int ldb, double* Q,
   if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
   if (r < 1) return -10;
   if (nq < 1) return -11;
-  if (int e = s2_check(r, nq)) return e;
+  if (int e = s2_check_variant(r, nq, flags)) return e;
   return run_host(kStage2, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, 2, nq, flags);
 }
 
@@ -1206,8 +1422,8 @@ int paraht_rht2ht_unblocked(int n, double* A, int lda, double* B, int ldb,
                             double* Q, int ldq, double* Z, int ldz, int r) {
   if (int e = check_common(n, lda, ldb, ldq, ldz)) return e;
   if (r < 1) return -10;
-  if (int e = s2_check(r, 1)) return e;
-  // one column per sweep: every rotation is applied before the next column
+  if (int e = s2_check_variant(r, 1, 0)) return e;
+  // one column per sweep: every reflector is applied before the next column
   return run_host(kStage2, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, 2, 1, 0);
 }
 
@@ -1218,7 +1434,7 @@ int paraht_reduce(int n, double* A, int lda, double* B, int ldb, double* Q,
   if (p < 2) return -11;
   if (nq < 1) return -12;
   if (!pht::s1_supported(r, p)) return PARAHT_ERR_UNSUPPORTED;
-  if (int e = s2_check(r, nq)) return e;  // before stage 1 changes anything
+  if (int e = s2_check_variant(r, nq, flags)) return e;  // before stage 1 changes anything
   return run_host(kBoth, n, A, lda, B, ldb, Q, ldq, Z, ldz, r, p, nq, flags);
 }
 
@@ -1240,7 +1456,7 @@ int paraht_rht2ht_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
   if (int e = check_common(n, lda, ldb, ldq, ldz)) return e - 1;
   if (r < 1) return -11;
   if (nq < 1) return -12;
-  if (int e = s2_check(r, nq)) return e;
+  if (int e = s2_check_variant(r, nq, flags)) return e;
   return run_dev(ctx, kStage2, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, 2, nq, flags);
 }
 
@@ -1253,7 +1469,7 @@ int paraht_reduce_dev(paraht_ctx* ctx, int n, double* dA, int lda, double* dB,
   if (p < 2) return -12;
   if (nq < 1) return -13;
   if (!pht::s1_supported(r, p)) return PARAHT_ERR_UNSUPPORTED;
-  if (int e = s2_check(r, nq)) return e;  // before stage 1 changes anything
+  if (int e = s2_check_variant(r, nq, flags)) return e;  // before stage 1 changes anything
   return run_dev(ctx, kBoth, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, p, nq, flags);
 }
 
@@ -1286,8 +1502,10 @@ int paraht_reduce_dev_noalloc(void* stream, int n, double* dA, int lda, double* 
   if (nq < 1) return -13;
   if (flags & ~PARAHT_STAGE2_GIVENS) return -14;  // per-phase timing needs events
   if (!pht::s1_supported(r, p)) return PARAHT_ERR_UNSUPPORTED;
-  if (int e = s2_check(r, nq)) return e;
-  const size_t need = std::max(s1_ws_bytes(n, r, p, 1), s2_ws_bytes(n, r, nq, 1));
+  if (int e = s2_check_variant(r, nq, flags)) return e;
+  const size_t need = std::max(s1_ws_bytes(n, r, p, 1), (flags & PARAHT_STAGE2_GIVENS)
+                                                            ? s2_ws_bytes(n, r, nq, 1)
+                                                            : s2h_ws_bytes(n, r, nq, 1));
   if (need > kScratchDoubles * sizeof(double)) {
     if (verbose())
       fprintf(stderr, "paraht: n=%d needs %zu bytes of workspace, the allocation-free path has %zu\n",
@@ -1311,7 +1529,7 @@ int paraht_reduce_dev_noalloc(void* stream, int n, double* dA, int lda, double* 
   c.ws = sd.ptr;
   c.ws_bytes = kScratchDoubles * sizeof(double);
   int st = stage1(&c, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, p, flags);
-  if (!st) st = stage2(&c, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, nq, flags);
+  if (!st) st = stage2_any(&c, n, dA, lda, dB, ldb, dQ, ldq, dZ, ldz, r, nq, flags);
   if (!st) CK(hipStreamSynchronize(c.stream));
   return st;
 }

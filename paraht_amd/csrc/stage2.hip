@@ -75,46 +75,6 @@ __global__ void k_s2_prep(double* CU, double* SU, double* CV, double* SV,
   if (gid == 0 && clear_err) *errflag = 0;
 }
 
-// Cross-workgroup hand-off.  An agent-scope acquire or release on gfx950
-// also invalidates (buffer_inv sc1) or writes back (buffer_wbl2 sc1) the
-// L2 of the XCD it runs on, which every other kernel on that XCD pays for.
-// So the spin reads the counter with relaxed agent-scope loads (coherent,
-// no invalidation per poll), thread 0 alone issues one acquire fence once
-// every wait is over (acquire_waits), and a step is published by one
-// release store from thread 0 after every wave's stores have completed
-// (publish).  The workgroup barrier between thread 0 and the other waves
-// carries the ordering to them.
-
-// Thread 0 spins (bounded by `limit` polls) until *p >= target; a wait that
-// gives up sets *errflag and lets the workgroup go on, so every wave exits.
-__device__ __forceinline__ void wait_at_least(int* p, int target, int* errflag, int limit) {
-  if (threadIdx.x == 0) {
-    int spins = 0;
-    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++spins > limit) {
-        __hip_atomic_store(errflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-    }
-  }
-}
-
-// After thread 0's waits: one agent-scope acquire, then the workgroup barrier.
-__device__ __forceinline__ void acquire_waits() {
-  if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  __syncthreads();
-}
-
-// Every wave waits for its own global stores to complete (reach L2), the
-// barrier orders that before thread 0's release store, whose L2 write-back
-// then carries all of them.
-__device__ __forceinline__ void publish(int* p, int value) {
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(p, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Helper workgroup h of column ic: after its column's step ib is published,
 // applies the step's right rotations to the window rows above the bulge,
 // rows istart..row0-1 of A and of B (rht2ht.f90:853-867,937-951), and

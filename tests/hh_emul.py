@@ -330,12 +330,16 @@ class Sweep:
     # -- the GPU's form of the row updates: per-row ranges of sequences
     def row_range(self, i, ib, phase):
         """Sequences [lo, hi] (1-based, empty if lo > hi) of block ib that row
-        i (1-based) receives in phase 'look' or 'far'."""
+        i (1-based) receives in phase 'look', 'far' or 'all' (both)."""
         r, nq, s = self.r, self.nq, self.s
         if i <= s:
-            return (1, nq) if phase == "far" else (1, 0)
+            return (1, 0) if phase == "look" else (1, nq)
         base = s + 1 + (ib - 1) * r
+        if i >= base:
+            return 1, 0
         d = (i - base) // r  # floor
+        if phase == "all":
+            return max(1, d + nq + 1), nq
         if phase == "look":
             return max(1, d + nq + 1), min(nq, d + 2 * nq)
         if d <= -2 * nq:
@@ -368,10 +372,13 @@ class Sweep:
             C -= W2 @ Vb.T
 
 
-def reduce_sweeps(A, B, Q, Z, r, nq, *, rng=None, masked=False, next_first=False):
+def reduce_sweeps(A, B, Q, Z, r, nq, *, rng=None, masked=False, next_first=False,
+                  one_pass=False):
     """Stage 2 by this module's sweeps (in place).  rng: random generation
     order; masked: the GPU form of the row updates; next_first: each sweep's
-    far updates after the next sweep's generation (and lookahead)."""
+    far updates after the next sweep's generation (and lookahead); one_pass:
+    the GPU's single-stream form, all row updates of a matrix in one masked
+    pass, then all its column updates."""
     n = A.shape[0]
     pending = None
     s = 1
@@ -379,6 +386,16 @@ def reduce_sweeps(A, B, Q, Z, r, nq, *, rng=None, masked=False, next_first=False
         sw = Sweep(A, B, Q, Z, r, nq, s)
         sw.generate(rng)
         sw.factors()
+        if one_pass:
+            for M in (B, A):
+                sw.rows_masked(M, "all")
+                for ib in range(sw.nblocks, 0, -1):
+                    r0, nb, istop = sw._left_geom(M, ib)
+                    if n >= istop + 1:
+                        sw._wy_left(M, ib, r0, nb, (istop, n))
+            sw.qz()
+            s += nq
+            continue
         if pending is not None:  # the previous sweep's far updates
             _far_all(pending, masked)
             pending = None

@@ -6,7 +6,7 @@ src/rht2ht.f90:1346-2051), on the numpy emulation in hh_emul.py:
     reference's column order, and a lag of 1 or asynchronous helpers at
     nq > r+1 are not (planted);
   * the per-row masked compact-WY row updates equal the reference's dlarf
-    edges + dlarfb bulk to rounding;
+    edges + dlarfb bulk to rounding, split (lookahead, rest) or in one pass;
   * sweep k+1's generation touches no entry that sweep k's far updates read or
     write (poisoned entries), so the far updates can run beside it.
 """
@@ -119,6 +119,19 @@ def test_masked_row_updates_match_reference_form(n, r, nq):
     mk = _copy(ms)
     H.reduce_sweeps(*mk, r, nq, masked=True)
     assert _maxdiff(em, mk) <= 1e-12 * np.linalg.norm(ms[0])
+
+
+@pytest.mark.parametrize("n,r,nq", CASES)
+def test_one_pass_row_then_column_updates_match(n, r, nq):
+    """The single-stream order: per matrix, every block's right reflectors on
+    each row over its whole range, then every block's left reflectors on the
+    columns right of the window."""
+    ms = _band(n, r)
+    em = _copy(ms)
+    H.reduce_sweeps(*em, r, nq)
+    op = _copy(ms)
+    H.reduce_sweeps(*op, r, nq, one_pass=True)
+    assert _maxdiff(em, op) <= 1e-12 * np.linalg.norm(ms[0])
 
 
 @pytest.mark.parametrize("n,r,nq", CASES)
