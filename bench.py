@@ -118,7 +118,9 @@ def cpu_baseline(args, gpu_cfg0_s):
 
 def bench_cfg0(args):
     """This build's GPU time for the reference README's configuration: one
-    pencil of n = 1000, r = 16, p = 8, nq = 8, resident in HBM (best of 3)."""
+    pencil of n = 1000, r = 16, p = 8, nq = 8, resident in HBM (best of 3 after
+    one untimed call), with the Householder stage 2 (the variant the README
+    times) and with the Givens one."""
     import torch
 
     import paraht_amd as ph
@@ -131,17 +133,20 @@ def bench_cfg0(args):
                         mode="r")[1]
     B0 = ph.column_major(torch.triu(R))
     ctx = ph.Context(0)
-    best = None
-    for _ in range(4):
-        A, B = A0.clone(), B0.clone()
-        Q = ph.column_major(torch.eye(n, dtype=torch.float64, device=dev))
-        Z = Q.clone()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ph.reduce(A, B, 16, 8, 8, Q, Z, ctx=ctx)
-        t = time.perf_counter() - t0
-        best = t if best is None else min(best, t)
-    return best
+    out = {}
+    for name, givens in (("householder", False), ("givens", True)):
+        best = None
+        for _ in range(4):
+            A, B = A0.clone(), B0.clone()
+            Q = ph.column_major(torch.eye(n, dtype=torch.float64, device=dev))
+            Z = Q.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ph.reduce(A, B, 16, 8, 8, Q, Z, ctx=ctx, givens=givens)
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+        out[name] = best
+    return out
 
 
 def bench_batched(args):
@@ -269,9 +274,10 @@ def main():
                     help="blocks per stage-1 reflector block (0: the tuned table, else 2)")
     ap.add_argument("--nq", type=int, default=0,
                     help="columns per stage-2 sweep (0: the tuned table, else 16)")
-    ap.add_argument("--stage2", choices=("householder", "givens"), default="householder",
-                    help="stage-2 variant: rht2ht_householder (the reference driver's) or "
-                         "rht2ht_givens")
+    ap.add_argument("--stage2", choices=("householder", "givens"), default="givens",
+                    help="stage-2 variant of the timed steps: rht2ht_givens (the faster chase "
+                         "here) or rht2ht_householder (the reference driver's); the other is "
+                         "timed once beside it")
     ap.add_argument("--cpu-n", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -281,6 +287,8 @@ def main():
                     help="pencils timed through the single-pencil path for the loop baseline")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward + backward measurement")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the reduction with the other stage-2 variant")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the HT reduction of one pencil is "
@@ -355,6 +363,28 @@ def main():
     zH = int(torch.count_nonzero(torch.tril(A, -2)).item())
     zT = int(torch.count_nonzero(torch.tril(B, -1)).item())
 
+    # the other stage-2 variant on the same pencil: one untimed call (its
+    # workspace), then one timed reduction
+    variants = {args.stage2: {"stage1_s": sum(t1s) / args.steps,
+                              "stage2_s": sum(t2s) / args.steps,
+                              "backward_errors": [ea, eb]}}
+    other = "householder" if givens else "givens"
+    if not args.no_other:
+        for rep in range(2):
+            A.copy_(A0)
+            B.copy_(B0)
+            Q.copy_(eye)
+            Z.copy_(eye)
+            torch.cuda.synchronize()
+            ph.reduce(A, B, r, p, nq, Q, Z, ctx=ctx, givens=not givens)
+        lt = ctx.last_times()
+        oa = (torch.linalg.norm(Q.T @ A0 @ Z - A) / torch.linalg.norm(A0)).item()
+        ob = (torch.linalg.norm(Q.T @ B0 @ Z - B) / torch.linalg.norm(B0)).item()
+        variants[other] = {"stage1_s": lt["stage1_ms"] / 1e3, "stage2_s": lt["stage2_ms"] / 1e3,
+                           "backward_errors": [oa, ob]}
+        print(f"{other}: stage2 {variants[other]['stage2_s']:.3f} s", file=sys.stderr,
+              flush=True)
+
     ms = wall / args.steps * 1e3
     t_red = (sum(t1s) + sum(t2s)) / args.steps
     flops = 14.0 * n ** 3
@@ -375,6 +405,7 @@ def main():
         "stage1_pct_fp64_peak": 100.0 * (28.0 / 3.0) * n ** 3 / t1 / FP64_PEAK,
         "stage2_hbm_gbs": 32.0 * n ** 3 / nq / t2 / 1e9,
         "backward_errors": [ea, eb], "nonzeros_below_structure": [zH, zT],
+        "stage2_variants": variants,
     }
     if kstats:
         tot = sum(v[0] for v in kstats.values()) or 1.0

@@ -31,8 +31,10 @@ extern "C" {
 #define PARAHT_TIME_COMPONENTS 0x1 /* print per-phase times like the reference's
                                        time_components argument */
 #define PARAHT_STAGE2_GIVENS 0x2   /* stage 2 with Givens rotations
-                                       (rht2ht_givens); default is the
-                                       Householder variant's entry point */
+                                       (rht2ht_givens); without it stage 2 is
+                                       the Householder variant
+                                       (rht2ht_householder, r <= 64, nq <= 32,
+                                       r + nq <= 96) */
 
 /* Opaque per-device context: stream, workspaces and phase timers.
  * A context is not re-entrant; use one per host thread. */

@@ -1,8 +1,10 @@
 ! Drop-in for the reference's module rht2ht (src/rht2ht.f90): the three
 ! stage-2 entry points with their names and argument lists
-! (src/rht2ht.f90:696-700, 1346-1351, 2053-2057).  All three run the GPU's
-! Givens bulge chase, which produces the same Hessenberg-triangular pencil
-! (unique up to the signs of rows and columns for an unreduced pencil).
+! (src/rht2ht.f90:696-700, 1346-1351, 2053-2057).  rht2ht_householder and
+! rht2ht_householder_unblocked run the GPU's Householder bulge chase (the
+! reference's reflectors, in its order), rht2ht_givens the GPU's Givens chase,
+! which reaches the same Hessenberg-triangular pencil up to the signs of rows
+! and columns.
 module rht2ht
    use paraht_capi
    implicit none
