@@ -393,7 +393,7 @@ __device__ inline void panel_t_body(double* sm, int n, int jb, int nb, int p, in
 // (the 'B','R' branch, ab2rht.f90:305-309).
 // RQ phase: thread t owns row t/8 of the block, columns 8*j + t%8 (j < 16).
 // Y / LQ phase: thread t owns row t/16 of Y = Q(0:nb-1,:), columns 16*j + t%16.
-__device__ inline void rq_gen_body(
+__device__ __forceinline__ void rq_gen_body(
     double* sm, const double* __restrict__ B, int ldb, int i0, int len, int nb,
     double* __restrict__ Vout, int ldv, double* __restrict__ Tout, int ldt) {
   double* Vr = sm;                  // [128][kLd]: RQ reflector i in row i
