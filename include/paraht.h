@@ -188,6 +188,43 @@ int paraht_adjoint_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils,
                        const double* dH, const double* dT, const double* dTinv, double* dGQ,
                        const double* dGZ, double* dXA, double* dXB);
 
+/* ---- QZ iteration on a Hessenberg-triangular pencil ----
+ * No counterpart in the reference: its driver stops at the HT form and checks
+ * it against LAPACK DGGHD3 (src/profilereduction.f90:152-162).  These take the
+ * place of the LAPACK routine that follows it, DHGEQZ(JOB, COMPQ, COMPZ, N,
+ * ILO=1, IHI=N, H, LDH, T, LDT, ALPHAR, ALPHAI, BETA, Q, LDQ, Z, LDZ, ...):
+ * the implicit double-shift QZ iteration on (H, T) with H upper Hessenberg
+ * and T upper triangular (only those parts are read).  The generalized
+ * eigenvalues are (alphar[j] + i alphai[j]) / beta[j]; beta[j] = 0 marks an
+ * infinite eigenvalue; beta >= 0; complex pairs are adjacent with
+ * alphai[j] > 0.
+ * Without PARAHT_QZ_SCHUR (DHGEQZ's JOB='E') only the eigenvalues are
+ * computed and H, T are left in an unspecified state.  With it (JOB='S')
+ * H becomes quasi-upper-triangular S (1x1 and 2x2 blocks, the latter for
+ * complex pairs, whose T block is diagonal and positive), T upper triangular
+ * P, and Q <- Q Qz, Z <- Z Zz when Q, Z are not NULL (COMPQ/COMPZ='V'), so
+ * that Q^T A Z = S and Q^T B Z = P for the pencil (A, B) whose reduction gave
+ * (H, T, Q, Z).  *info: 0, or the 1-based index of the last eigenvalue that
+ * did not converge within 30 n sweeps (DHGEQZ's INFO = 1..N); eigenvalues
+ * info+1..n are valid then (a pencil with NaN or Inf entries returns n at
+ * once).  One workgroup per pencil on the GPU. */
+#define PARAHT_QZ_SCHUR 0x8
+int paraht_hgeqz(int n, double* H, int ldh, double* T, int ldt, double* Q, int ldq, double* Z,
+                 int ldz, double* alphar, double* alphai, double* beta, int flags, int* info);
+/* The same on device arrays (alphar, alphai, beta: n doubles each in HBM;
+ * info on the host). */
+int paraht_hgeqz_dev(paraht_ctx* ctx, int n, double* dH, int ldh, double* dT, int ldt,
+                     double* dQ, int ldq, double* dZ, int ldz, double* dalphar, double* dalphai,
+                     double* dbeta, int flags, int* info);
+/* Many pencils in one launch, packed as for paraht_batched_dev (pencil i's
+ * n x n matrices at pencils[i].offset of each pool, ld = n; r, p, nq are
+ * ignored; dQ, dZ may be NULL).  Pencil i's eigenvalues are at element
+ * offset n_0 + ... + n_{i-1} of dalphar, dalphai, dbeta; info[i] (host array
+ * of count ints) is its status. */
+int paraht_hgeqz_batched_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils,
+                             double* dH, double* dT, double* dQ, double* dZ, double* dalphar,
+                             double* dalphai, double* dbeta, int flags, int* info);
+
 /* Library version (major*10000 + minor*100 + patch). */
 int paraht_version(void);
 /* Human-readable text of a status code. */

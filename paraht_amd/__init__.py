@@ -32,7 +32,7 @@ __all__ = [
     "ab2rht_simple", "rht2ht_householder", "rht2ht_givens",
     "rht2ht_householder_unblocked", "reduce", "qr_b", "reduce_general", "Context",
     "ParahtError",
-    "build", "column_major", "batched", "adjoint",
+    "build", "column_major", "batched", "adjoint", "qz", "hgeqz",
 ]
 
 build = _lib.build
@@ -276,3 +276,5 @@ def reduce_general(A, B, r, p, nq, Q=None, Z=None, qr_nb=64, qr_p=2,
 
 from . import batched  # noqa: E402  (batched reduction of many small pencils)
 from . import adjoint  # noqa: E402  (reverse mode: ht_backward, ht_reduce)
+from . import qz  # noqa: E402  (QZ iteration on the reduced pencil)
+from .qz import hgeqz  # noqa: E402

@@ -16,6 +16,7 @@ _lib = None
 PARAHT_TIME_COMPONENTS = 0x1
 PARAHT_STAGE2_GIVENS = 0x2
 PARAHT_BATCH_GRAPH = 0x4
+PARAHT_QZ_SCHUR = 0x8
 
 
 class Pencil(ctypes.Structure):
@@ -53,6 +54,10 @@ SIGNATURES = {
                                  ctypes.POINTER(_I)], _I),
     "paraht_adjoint_dev": ([_P, _I, ctypes.POINTER(Pencil), _DP, _DP, _DP, _DP, _DP, _DP, _DP],
                            _I),
+    "paraht_hgeqz": (_SIG_HOST12[:9] + [_DP, _DP, _DP, _I, ctypes.POINTER(_I)], _I),
+    "paraht_hgeqz_dev": ([_P] + _SIG_HOST12[:9] + [_DP, _DP, _DP, _I, ctypes.POINTER(_I)], _I),
+    "paraht_hgeqz_batched_dev": ([_P, _I, ctypes.POINTER(Pencil), _DP, _DP, _DP, _DP, _DP, _DP,
+                                  _DP, _I, ctypes.POINTER(_I)], _I),
     "paraht_version": ([], _I),
     "paraht_status_string": ([_I], ctypes.c_char_p),
 }

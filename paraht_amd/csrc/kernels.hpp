@@ -263,4 +263,21 @@ hipError_t adjoint_grid(hipStream_t st, int n, const double* H, const double* T,
 // Largest co-resident grid of the cooperative adjoint kernel (0 on failure).
 int adjoint_grid_max_blocks(int device);
 
+// ---- QZ iteration on Hessenberg-triangular pencils ----
+// One task per pencil: H, T, Q, Z at element offset `off` of their pools
+// (leading dimensions ldh..ldz), eigenvalues at `eoff` of alphar/alphai/beta,
+// status at dinfo[idx].
+struct QzTask {
+  long long off, eoff;
+  int n, ldh, ldt, ldq, ldz, idx;
+};
+constexpr int kQzSchur = 0x1;  // Schur form (whole rows/columns, Q and Z)
+// Workgroup size for a pencil of order n.
+int qz_threads(int n);
+// One workgroup per task; dinfo[idx] = 0 or the 1-based index of the last
+// eigenvalue that did not converge in 30 n sweeps.
+hipError_t qz_launch(hipStream_t st, const QzTask* dtasks, int count, int threads, double* H,
+                     double* T, double* Q, double* Z, double* ar, double* ai, double* be,
+                     int* dinfo, int flags);
+
 }  // namespace pht

@@ -1271,6 +1271,51 @@ int adjoint(paraht_ctx* c, int count, const paraht_pencil* pencils, const double
 }  // namespace
 
 // ------------------------------------------------------------------------
+// QZ iteration
+// ------------------------------------------------------------------------
+namespace {
+
+int qz_threads_for(int nmax, int count) {
+  if (const char* e = getenv("PARAHT_QZ_THREADS")) {
+    const int t = atoi(e);
+    if (t >= 64 && t <= 1024 && t % 64 == 0) return t;
+  }
+  return count == 1 ? pht::qz_threads(nmax) : 256;
+}
+
+// tasks in the caller's order (idx = position); runs them largest first and
+// writes each pencil's status to info[idx] (host)
+int qz_run(paraht_ctx* c, std::vector<pht::QzTask>& q, double* H, double* T, double* Q,
+           double* Z, double* ar, double* ai, double* be, int flags, int* info) {
+  const int count = (int)q.size();
+  for (int i = 0; i < count; ++i) info[i] = 0;
+  std::vector<pht::QzTask> run;
+  run.reserve(count);
+  for (const auto& t : q)
+    if (t.n > 0) run.push_back(t);
+  if (run.empty()) return 0;
+  std::stable_sort(run.begin(), run.end(),
+                   [](const pht::QzTask& a, const pht::QzTask& b) { return a.n > b.n; });
+  const int nrun = (int)run.size();
+  const size_t tbytes = ((nrun * sizeof(pht::QzTask) + 255) / 256) * 256;
+  if (int e = ensure_ws(c, tbytes + (size_t)count * sizeof(int))) return e;
+  auto* dT = (pht::QzTask*)c->ws;
+  int* dinfo = (int*)((char*)c->ws + tbytes);
+  hipStream_t st = c->stream;
+  CK(hipMemsetAsync(dinfo, 0, (size_t)count * sizeof(int), st));
+  CK(hipMemcpyAsync(dT, run.data(), nrun * sizeof(pht::QzTask), hipMemcpyHostToDevice, st));
+  const int kflags = (flags & PARAHT_QZ_SCHUR) ? pht::kQzSchur : 0;
+  CK(pht::qz_launch(st, dT, nrun, qz_threads_for(run[0].n, nrun), H, T,
+                    (flags & PARAHT_QZ_SCHUR) ? Q : nullptr, (flags & PARAHT_QZ_SCHUR) ? Z : nullptr,
+                    ar, ai, be, dinfo, kflags));
+  CK(hipMemcpyAsync(info, dinfo, (size_t)count * sizeof(int), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));  // run (a host temporary) and info
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
 // Allocation-free reduction
 // ------------------------------------------------------------------------
 // The workspace of paraht_reduce_dev_noalloc: kScratchDoubles FP64 words
@@ -1564,6 +1609,98 @@ int paraht_adjoint_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils,
   if (!dXB) return -10;
   CK(hipSetDevice(ctx->device));
   return adjoint(ctx, count, pencils, dH, dT, dTinv, dGQ, dGZ, dXA, dXB);
+}
+
+int paraht_hgeqz_dev(paraht_ctx* ctx, int n, double* dH, int ldh, double* dT, int ldt,
+                     double* dQ, int ldq, double* dZ, int ldz, double* dalphar, double* dalphai,
+                     double* dbeta, int flags, int* info) {
+  if (!ctx) return -1;
+  if (n < 0) return -2;
+  if (!dH && n > 0) return -3;
+  if (ldh < std::max(1, n)) return -4;
+  if (!dT && n > 0) return -5;
+  if (ldt < std::max(1, n)) return -6;
+  const bool schur = (flags & PARAHT_QZ_SCHUR) != 0;
+  if (schur && dQ && ldq < std::max(1, n)) return -8;
+  if (schur && dZ && ldz < std::max(1, n)) return -10;
+  if ((!dalphar || !dalphai || !dbeta) && n > 0) return -11;
+  if (flags & ~PARAHT_QZ_SCHUR) return -14;
+  if (!info) return -15;
+  *info = 0;
+  if (n == 0) return 0;
+  CK(hipSetDevice(ctx->device));
+  std::vector<pht::QzTask> q(1);
+  q[0] = {0, 0, n, ldh, ldt, dQ ? ldq : n, dZ ? ldz : n, 0};
+  return qz_run(ctx, q, dH, dT, dQ, dZ, dalphar, dalphai, dbeta, flags, info);
+}
+
+int paraht_hgeqz_batched_dev(paraht_ctx* ctx, int count, const paraht_pencil* pencils,
+                             double* dH, double* dT, double* dQ, double* dZ, double* dalphar,
+                             double* dalphai, double* dbeta, int flags, int* info) {
+  if (!ctx) return -1;
+  if (count < 0) return -2;
+  if (count == 0) return 0;
+  if (!pencils) return -3;
+  if (!dH) return -4;
+  if (!dT) return -5;
+  if (!dalphar || !dalphai || !dbeta) return -8;
+  if (flags & ~PARAHT_QZ_SCHUR) return -11;
+  if (!info) return -12;
+  std::vector<pht::QzTask> q(count);
+  long long eoff = 0;
+  for (int i = 0; i < count; ++i) {
+    const paraht_pencil& pc = pencils[i];
+    if (pc.n < 0 || pc.offset < 0) return -3;
+    q[i] = {pc.offset, eoff, pc.n, std::max(pc.n, 1), std::max(pc.n, 1), std::max(pc.n, 1),
+            std::max(pc.n, 1), i};
+    eoff += pc.n;
+  }
+  CK(hipSetDevice(ctx->device));
+  return qz_run(ctx, q, dH, dT, dQ, dZ, dalphar, dalphai, dbeta, flags, info);
+}
+
+int paraht_hgeqz(int n, double* H, int ldh, double* T, int ldt, double* Q, int ldq, double* Z,
+                 int ldz, double* alphar, double* alphai, double* beta, int flags, int* info) {
+  if (n < 0) return -1;
+  if (!H && n > 0) return -2;
+  if (ldh < std::max(1, n)) return -3;
+  if (!T && n > 0) return -4;
+  if (ldt < std::max(1, n)) return -5;
+  const bool schur = (flags & PARAHT_QZ_SCHUR) != 0;
+  if (schur && Q && ldq < std::max(1, n)) return -7;
+  if (schur && Z && ldz < std::max(1, n)) return -9;
+  if ((!alphar || !alphai || !beta) && n > 0) return -10;
+  if (flags & ~PARAHT_QZ_SCHUR) return -13;
+  if (!info) return -14;
+  *info = 0;
+  if (n == 0) return 0;
+  paraht_ctx* c = nullptr;
+  if (int e = default_ctx(&c)) return e;
+  hipStream_t st = c->stream;
+  const bool vq = schur && Q, vz = schur && Z;
+  DevMat dH, dT, dQ, dZ, dE;
+  if (int e = upload(st, dH, H, n, ldh)) return e;
+  if (int e = upload(st, dT, T, n, ldt)) return e;
+  if (vq)
+    if (int e = upload(st, dQ, Q, n, ldq)) return e;
+  if (vz)
+    if (int e = upload(st, dZ, Z, n, ldz)) return e;
+  CK(hipMalloc(&dE.d, 3 * (size_t)n * sizeof(double)));
+  std::vector<pht::QzTask> q(1);
+  q[0] = {0, 0, n, n, n, n, n, 0};
+  if (int e = qz_run(c, q, dH.d, dT.d, dQ.d, dZ.d, dE.d, dE.d + n, dE.d + 2 * n, flags, info))
+    return e;
+  if (int e = download(st, H, ldh, dH, n)) return e;
+  if (int e = download(st, T, ldt, dT, n)) return e;
+  if (vq)
+    if (int e = download(st, Q, ldq, dQ, n)) return e;
+  if (vz)
+    if (int e = download(st, Z, ldz, dZ, n)) return e;
+  CK(hipMemcpyAsync(alphar, dE.d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(alphai, dE.d + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(beta, dE.d + 2 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  return 0;
 }
 
 int paraht_batched_defaults(int n, int* r, int* p, int* nq) {

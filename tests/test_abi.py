@@ -80,3 +80,26 @@ def test_torch_shapes_smaller_than_n_are_rejected():
     with pytest.raises(ValueError, match="smaller than n x n"):
         ph._check_shapes(n, [(A, None), (torch.zeros(n, n - 1), None)], ("A", "B"), "x")
     ph._check_shapes(n, [(A, None), (torch.zeros(n + 2, n), None)], ("A", "B"), "x")
+
+
+def test_qz_argument_errors_are_reported_without_a_device():
+    """paraht_hgeqz checks its arguments before any device work, counting
+    them like DHGEQZ's LAPACK-style -i (in the order of include/paraht.h)."""
+    import ctypes
+    import numpy as np
+    from paraht_amd import _lib
+    L = _lib.lib()
+    H = np.zeros((4, 4), order="F")
+    p = ctypes.c_void_p(H.ctypes.data)
+    info = ctypes.c_int(7)
+    ev = [ctypes.c_void_p(np.zeros(4).ctypes.data) for _ in range(3)]
+    assert L.paraht_hgeqz(-1, p, 4, p, 4, None, 1, None, 1, *ev, 0, ctypes.byref(info)) == -1
+    assert L.paraht_hgeqz(4, p, 3, p, 4, None, 1, None, 1, *ev, 0, ctypes.byref(info)) == -3
+    assert L.paraht_hgeqz(4, p, 4, p, 2, None, 1, None, 1, *ev, 0, ctypes.byref(info)) == -5
+    assert L.paraht_hgeqz(4, p, 4, p, 4, p, 2, None, 1, *ev, _lib.PARAHT_QZ_SCHUR,
+                          ctypes.byref(info)) == -7
+    assert L.paraht_hgeqz(4, p, 4, p, 4, None, 1, None, 1, *ev, 0x40, ctypes.byref(info)) == -13
+    assert L.paraht_hgeqz(4, p, 4, p, 4, None, 1, None, 1, *ev, 0, None) == -14
+    # n = 0 is a quick return with info = 0
+    assert L.paraht_hgeqz(0, None, 1, None, 1, None, 1, None, 1, None, None, None, 0,
+                          ctypes.byref(info)) == 0 and info.value == 0
