@@ -195,6 +195,8 @@ def bench_batched(args):
     t0 = time.perf_counter()
     for i in range(k):
         r, p, nq = defaults(sizes[i])
+        if r == 1:  # the one-stage path exists only in the batched kernel
+            r, p, nq = 8, 2, 4
         ph.reduce(ph.column_major(As[i].clone()), ph.column_major(Bs[i].clone()), r, p, nq)
     torch.cuda.synchronize()
     t_l = time.perf_counter() - t0
@@ -263,6 +265,94 @@ def bench_adjoint(args):
             "forward_s": t_f, "forward_backward_s": t_fb, "ratio": t_fb / t_f}
 
 
+def bench_qz(args):
+    """The QZ iteration after the reduction (paraht_hgeqz*): generalized
+    eigenvalues of (1) single pencils of order 1000 and 2000 (A ~ N(0,1),
+    B = R of the QR of an N(0,1) matrix, reduced on the GPU first, untimed),
+    eigenvalues only and the Schur form with Q, Z; (2) the batched draw of
+    2048 pencils with n uniform in [32, 512] after their batched reduction,
+    eigenvalues only, in one launch.  CPU baseline: LAPACK DHGEQZ (scipy's
+    OpenBLAS) on the order-1000 pencil and on every 64th pencil of the
+    batch."""
+    import numpy as np
+    import torch
+
+    import paraht_amd as ph
+    from paraht_amd.batched import Batch, reduce_batch
+    from paraht_amd.qz import hgeqz_batch
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(77)
+    out = {"single": []}
+    host = {}
+    for n in (1000, 2000):
+        A = ph.column_major(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+        B = ph.column_major(torch.triu(torch.linalg.qr(
+            torch.randn(n, n, dtype=torch.float64, device=dev, generator=g), mode="r")[1]))
+        H, T, Q, Z = ph.reduce(A, B, 16, 4, 8, givens=True)
+        host[n] = (H.cpu().numpy(), T.cpu().numpy())
+        rec = {"n": n}
+        for mode in ("eigenvalues", "schur_qz"):
+            h, t = ph.column_major(H.clone()), ph.column_major(T.clone())
+            q, z = ph.column_major(Q.clone()), ph.column_major(Z.clone())
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if mode == "eigenvalues":
+                *_, info = ph.hgeqz(h, t)
+            else:
+                *_, info = ph.hgeqz(h, t, q, z, schur=True)
+            torch.cuda.synchronize()
+            rec[mode + "_s"] = time.perf_counter() - t0
+            rec[mode + "_info"] = info
+        print(f"qz: n={n} eigenvalues {rec['eigenvalues_s']:.3f} s, Schur+Q,Z "
+              f"{rec['schur_qz_s']:.3f} s", file=sys.stderr, flush=True)
+        out["single"].append(rec)
+    rng = np.random.default_rng(1302)
+    sizes = [int(x) for x in rng.integers(32, 513, size=2048)]
+    bt = Batch(sizes)
+    As = [torch.randn(n, n, dtype=torch.float64, device=dev, generator=g) for n in sizes]
+    Bs = [torch.triu(torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev,
+                                                 generator=g), mode="r")[1]) for n in sizes]
+    bt.load(As, Bs)
+    reduce_batch(bt)
+    H0, T0 = bt.A.clone(), bt.B.clone()
+    ts = []
+    for _ in range(2):
+        bt.A.copy_(H0)
+        bt.B.copy_(T0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        *_, info = hgeqz_batch(bt)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_b = min(ts)
+    out["batched"] = {"pencils": len(sizes), "n_range": [32, 512], "s": t_b,
+                      "pencils_per_s": len(sizes) / t_b, "nonconverged": sum(1 for x in info if x)}
+    print(f"qz: batched {len(sizes)} pencils in {t_b:.3f} s", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+        import oracle
+        from threadpoolctl import threadpool_limits
+        Hh, Th = host[1000]
+        sample = list(range(0, len(sizes), 64))
+        pencils = [(bt.view(H0, i).cpu().numpy(), bt.view(T0, i).cpu().numpy()) for i in sample]
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            oracle.dhgeqz(Hh, Th)
+            t_c = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            for Hs, Ts in pencils:
+                oracle.dhgeqz(Hs, Ts)
+            t_s = time.perf_counter() - t0
+        share = sum(sizes[i] ** 3 for i in sample) / (sum(n ** 3 for n in sizes) / len(sizes)
+                                                      * len(sample))
+        out["cpu_dhgeqz"] = {"routine": "LAPACK DHGEQZ JOB='E' (scipy OpenBLAS)", "cores": 1,
+                             "n1000_s": t_c,
+                             "batched_sample": len(sample),
+                             "batched_pencils_per_s": len(sample) / t_s * share}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,6 +377,8 @@ def main():
                     help="pencils timed through the single-pencil path for the loop baseline")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward + backward measurement")
+    ap.add_argument("--no-qz", action="store_true",
+                    help="skip the QZ iteration measurements")
     ap.add_argument("--no-other", action="store_true",
                     help="skip the reduction with the other stage-2 variant")
     args = ap.parse_args()
@@ -444,6 +536,8 @@ def main():
         out["batched"] = bench_batched(args)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args)
+    if not args.no_qz:
+        out["qz"] = bench_qz(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, bench_cfg0(args))
     print(json.dumps(out), flush=True)

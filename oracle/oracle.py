@@ -169,6 +169,70 @@ def dgghd3(A0, B0):
     return A, B, Q, Z
 
 
+def dgghrd(A0, B0):
+    """LAPACK DGGHRD('I','I',n,1,n,...) on copies (B0 upper triangular): the
+    unblocked Givens reduction to HT form; returns H, T, Q, Z.  The pin of
+    the batched path's one-stage kernel for tiny pencils, which applies the
+    same rotations in the same order."""
+    L = _lapack()
+    f = L.scipy_dgghrd_
+    n = A0.shape[0]
+    A = np.array(A0, dtype=np.float64, order="F")
+    B = np.array(B0, dtype=np.float64, order="F")
+    Q = np.zeros((max(n, 1), max(n, 1)), order="F")
+    Z = np.zeros((max(n, 1), max(n, 1)), order="F")
+    ci = ctypes.c_int
+    nn, ilo, ihi, ld = ci(n), ci(1), ci(n), ci(max(1, n))
+    info = ci(0)
+    cq = ctypes.c_char(b"I")
+    f(ctypes.byref(cq), ctypes.byref(cq), ctypes.byref(nn), ctypes.byref(ilo),
+      ctypes.byref(ihi), _ptr(A), ctypes.byref(ld), _ptr(B), ctypes.byref(ld),
+      _ptr(Q), ctypes.byref(ld), _ptr(Z), ctypes.byref(ld), ctypes.byref(info),
+      ctypes.c_size_t(1), ctypes.c_size_t(1))
+    if info.value != 0:
+        raise RuntimeError(f"dgghrd info={info.value}")
+    return A, B, Q[:n, :n], Z[:n, :n]
+
+
+def dhgeqz(H0, T0, schur=False, Q0=None, Z0=None):
+    """LAPACK DHGEQZ(JOB='E' or 'S', COMPQ/COMPZ='N' or 'V', ILO=1, IHI=n) on
+    copies of a Hessenberg-triangular pencil; returns (H, T, Q, Z, alphar,
+    alphai, beta, info).  The library routine the GPU QZ iteration takes the
+    place of (the reference's driver stops before it,
+    src/profilereduction.f90:152-162)."""
+    L = _lapack()
+    f = L.scipy_dhgeqz_
+    n = H0.shape[0]
+    H = np.array(H0, dtype=np.float64, order="F")
+    T = np.array(T0, dtype=np.float64, order="F")
+    vq, vz = schur and Q0 is not None, schur and Z0 is not None
+    Q = np.array(Q0, dtype=np.float64, order="F") if vq else np.zeros((1, 1), order="F")
+    Z = np.array(Z0, dtype=np.float64, order="F") if vz else np.zeros((1, 1), order="F")
+    ar, ai, be = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    ci = ctypes.c_int
+    job = ctypes.c_char(b"S" if schur else b"E")
+    cq = ctypes.c_char(b"V" if vq else b"N")
+    cz = ctypes.c_char(b"V" if vz else b"N")
+    nn, ilo, ihi, ld = ci(n), ci(1), ci(n), ci(max(1, n))
+    ldq, ldz = ci(max(1, n) if vq else 1), ci(max(1, n) if vz else 1)
+    info = ci(0)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    tail = (ctypes.c_size_t(1), ctypes.c_size_t(1), ctypes.c_size_t(1))
+    for lwork in (-1, None):
+        if lwork is None:
+            lw = max(1, n, int(work[0]))
+            work = np.zeros(lw)
+            lwv = ci(lw)
+        else:
+            work = np.zeros(1)
+            lwv = ci(-1)
+        f(ctypes.byref(job), ctypes.byref(cq), ctypes.byref(cz), ctypes.byref(nn),
+          ctypes.byref(ilo), ctypes.byref(ihi), _ptr(H), ctypes.byref(ld), _ptr(T),
+          ctypes.byref(ld), dp(ar), dp(ai), dp(be), _ptr(Q), ctypes.byref(ldq), _ptr(Z),
+          ctypes.byref(ldz), dp(work), ctypes.byref(lwv), ctypes.byref(info), *tail)
+    return H, T, (Q if vq else None), (Z if vz else None), ar[:n], ai[:n], be[:n], info.value
+
+
 def qr_b(A0, B0, Q0=None):
     """QR of a general B through the host's LAPACK (scipy): B0 = Q_B R with
     DGEQRF; returns (Q_B^T A0, R with exact zeros below the diagonal,

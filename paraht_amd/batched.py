@@ -16,8 +16,12 @@ from ._lib import PARAHT_BATCH_GRAPH, Pencil
 # (r, nq) pairs and p values the batched kernel supports (r <= 16, p*r <= 128)
 PAIRS = [(8, 4), (8, 8), (16, 4), (16, 8), (16, 16)]
 P_VALUES = [2, 4, 8]
+# r = 1: the one-stage reduction (DGGHRD's Givens sequence with the pencil in
+# LDS) for pencils of order <= ONE_STAGE_MAX; p and nq are not used
+ONE_STAGE = (1, 2, 1)
+ONE_STAGE_MAX = 64
 # size buckets of n (upper bounds) the defaults and the autotuner use
-BUCKETS = [96, 192, 320, 512]
+BUCKETS = [64, 96, 192, 320, 512]
 
 
 def defaults(n: int):
@@ -92,7 +96,8 @@ def reduce_batch(batch: Batch, params=None, ctx=None, graph=False, desc=None):
 def autotune(sample, seconds_per_candidate=None, ctx=None, candidates=None):
     """Size-bucket autotuner: for each bucket of BUCKETS, times the batched
     launch on the sample pencils of that bucket with every supported
-    (r, p, nq) and returns {bucket_upper_bound: (r, p, nq)} of the fastest.
+    (r, p, nq) (the one-stage path too in buckets of order <= ONE_STAGE_MAX)
+    and returns {bucket_upper_bound: (r, p, nq)} of the fastest.
     sample: list of (A, B) host or device arrays."""
     import torch
     cands = candidates or [(r, p, nq) for (r, nq) in PAIRS for p in P_VALUES if p * r <= 128]
@@ -104,7 +109,10 @@ def autotune(sample, seconds_per_candidate=None, ctx=None, candidates=None):
         if not sub:
             continue
         best = None
-        for c in cands:
+        bucket = list(cands)
+        if candidates is None and hi <= ONE_STAGE_MAX:
+            bucket.append(ONE_STAGE)
+        for c in bucket:
             bt = Batch([a.shape[0] for a, _ in sub])
             bt.load([a for a, _ in sub], [b for _, b in sub])
             torch.cuda.synchronize()

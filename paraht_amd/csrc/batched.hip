@@ -14,7 +14,8 @@
 //    with register windows.
 // Every pencil has its own (r, p, nq), normally from the size-bucket table
 // (batched_defaults) or the autotuner; r <= 16, p*r <= 128 and (r, nq) one of
-// the pairs of PHT_BATCH_PAIRS.
+// the pairs of PHT_BATCH_PAIRS, or r = 1: the one-stage reduction of a pencil
+// of order <= 64 in LDS (onestage_wg).
 #include "apply_dev.hpp"
 #include "s1_dev.hpp"
 #include "s2_dev.hpp"
@@ -287,6 +288,117 @@ __device__ __noinline__ void stage2_wg(double* sm, double* A, double* B, double*
   }
 }
 
+// ---- one stage for tiny pencils (r = 1) ---------------------------------
+// A pencil of order n <= kOneStageMax is reduced straight to HT form with
+// LAPACK DGGHRD's Givens sequence (Moler & Stewart 1973): for each column j,
+// from the bottom, a left rotation of rows i-1, i zeroes A(i, j) and the fill
+// B(i, i-1) it makes is removed by a right rotation of columns i-1, i.  The
+// whole pencil sits in LDS (ld = n + 1).  Wave 0 runs the rotation chain on
+// A and B (one lane per row or column, no barrier inside a column); waves 1
+// and 2 apply the previous column's rotations to Q and Z from a
+// double-buffered LDS tape.  Rotations with dlartg's conventions (rot_rsq),
+// so the result equals DGGHRD's up to rounding.
+constexpr int kOneStageMax = 64;
+
+__device__ __forceinline__ void lds_order() { __asm__ volatile("" ::: "memory"); }
+
+__device__ __noinline__ void onestage_wg(double* sm, double* A, double* B, double* Q, double* Z,
+                                         int n) {
+  const int ld = n + 1;
+  double* a = sm;
+  double* b = a + (size_t)n * ld;
+  double* q = b + (size_t)n * ld;
+  double* z = q + (size_t)n * ld;
+  double* tape = z + (size_t)n * ld;  // [2 buffers][4][n]: cl, sl, cr, sr
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  for (int x = tid; x < n * n; x += kBT) {
+    const int i = x % n, j = x / n;
+    a[i + j * ld] = A[x];
+    b[i + j * ld] = B[x];
+    q[i + j * ld] = Q[x];
+    z[i + j * ld] = Z[x];
+  }
+  __syncthreads();
+  for (int j = 0; j <= n - 2; ++j) {
+    double* tp = tape + (size_t)(j & 1) * 4 * n;  // column j's rotations
+    double* tq = tape + (size_t)((j + 1) & 1) * 4 * n;  // column j-1's
+    if (wave == 0 && j <= n - 3) {
+      for (int i = n - 1; i >= j + 2; --i) {
+        double c, s, r;
+        rot_rsq(a[i - 1 + j * ld], a[i + j * ld], c, s, r);
+        lds_order();
+        if (l > j && l < n) {  // columns j+1.. of A, rows i-1, i
+          const double x = a[i - 1 + l * ld], y = a[i + l * ld];
+          a[i - 1 + l * ld] = c * x + s * y;
+          a[i + l * ld] = c * y - s * x;
+        }
+        if (l >= i - 1 && l < n) {  // columns i-1.. of B
+          const double x = b[i - 1 + l * ld], y = b[i + l * ld];
+          b[i - 1 + l * ld] = c * x + s * y;
+          b[i + l * ld] = c * y - s * x;
+        }
+        if (l == 0) {
+          a[i - 1 + j * ld] = r;
+          a[i + j * ld] = 0.0;
+          tp[i] = c;
+          tp[n + i] = s;
+        }
+        lds_order();
+        double c2, s2, r2;
+        rot_rsq(b[i + i * ld], b[i + (i - 1) * ld], c2, s2, r2);
+        lds_order();
+        if (l < n) {  // rows of A, columns i (x) and i-1 (y)
+          const double x = a[l + i * ld], y = a[l + (i - 1) * ld];
+          a[l + i * ld] = c2 * x + s2 * y;
+          a[l + (i - 1) * ld] = c2 * y - s2 * x;
+        }
+        if (l < i) {  // rows 0..i-1 of B
+          const double x = b[l + i * ld], y = b[l + (i - 1) * ld];
+          b[l + i * ld] = c2 * x + s2 * y;
+          b[l + (i - 1) * ld] = c2 * y - s2 * x;
+        }
+        if (l == 0) {
+          b[i + i * ld] = r2;
+          b[i + (i - 1) * ld] = 0.0;
+          tp[2 * n + i] = c2;
+          tp[3 * n + i] = s2;
+        }
+        lds_order();
+      }
+    } else if ((wave == 1 || wave == 2) && j >= 1 && j - 1 <= n - 3 && l < n) {
+      // column j-1's rotations on the rows of Q (wave 1) or Z (wave 2)
+      const int jp = j - 1;
+      if (wave == 1) {
+        for (int i = n - 1; i >= jp + 2; --i) {
+          const double c = tq[i], s = tq[n + i];
+          const double x = q[l + (i - 1) * ld], y = q[l + i * ld];
+          q[l + (i - 1) * ld] = c * x + s * y;
+          q[l + i * ld] = c * y - s * x;
+        }
+      } else {
+        for (int i = n - 1; i >= jp + 2; --i) {
+          const double c = tq[2 * n + i], s = tq[3 * n + i];
+          const double x = z[l + i * ld], y = z[l + (i - 1) * ld];
+          z[l + i * ld] = c * x + s * y;
+          z[l + (i - 1) * ld] = c * y - s * x;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int x = tid; x < n * n; x += kBT) {
+    const int i = x % n, j = x / n;
+    A[x] = a[i + j * ld];
+    B[x] = b[i + j * ld];
+    Q[x] = q[i + j * ld];
+    Z[x] = z[i + j * ld];
+  }
+}
+
+static_assert((4 * kOneStageMax * (kOneStageMax + 1) + 8 * kOneStageMax) * sizeof(double) <=
+                  s1::kRqLds,
+              "a tiny pencil fits the batched kernel's LDS");
+
 __global__ __launch_bounds__(kBT) void k_batched(const BatchPencil* __restrict__ pencils,
                                                  int count, int* next, double* Ap, double* Bp,
                                                  double* Qp, double* Zp, double* ws_all,
@@ -305,8 +417,12 @@ __global__ __launch_bounds__(kBT) void k_batched(const BatchPencil* __restrict__
     double* B = Bp + pc.offset;
     double* Q = Qp + pc.offset;
     double* Z = Zp + pc.offset;
-    if (pc.n > pc.r + 1) stage1_wg(sm, A, B, Q, Z, pc.n, pc.r, pc.p, ws);
-    if (pc.n > 2) stage2_wg(sm, A, B, Q, Z, pc.n, pc.r, pc.nq, ws);
+    if (pc.r == 1) {
+      onestage_wg(sm, A, B, Q, Z, pc.n);
+    } else {
+      if (pc.n > pc.r + 1) stage1_wg(sm, A, B, Q, Z, pc.n, pc.r, pc.p, ws);
+      if (pc.n > 2) stage2_wg(sm, A, B, Q, Z, pc.n, pc.r, pc.nq, ws);
+    }
     __threadfence();
     __syncthreads();
   }
@@ -318,6 +434,7 @@ __global__ void k_batch_reset(int* next) {
 }  // namespace
 
 bool batched_supported(int n, int r, int p, int nq) {
+  if (r == 1) return n >= 1 && n <= kOneStageMax;  // one stage; p, nq unused
   if (n < 1 || r < 1 || r > kBRmax || p < 2 || p * r > 128) return false;
   bool ok = false;
 #define PHT_OK(RR, NN) ok = ok || (r == RR && nq == NN);
@@ -327,6 +444,7 @@ bool batched_supported(int n, int r, int p, int nq) {
 }
 
 size_t batched_ws_doubles(int n, int r, int p, int nq) {
+  if (r == 1) return 64;
   const size_t P = (size_t)p * r;
   const size_t nbmax = (n > r) ? (size_t)((n - r + (p - 1) * r - 1) / ((p - 1) * r) + 1) : 1;
   const size_t s1 = 2 * nbmax * P * r + 2 * nbmax * r * r;

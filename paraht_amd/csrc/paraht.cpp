@@ -1129,10 +1129,18 @@ namespace {
 
 // size-bucket table of (r, p, nq) for the batched path, from
 // paraht_amd.batched.autotune on MI355X over 2048 pencils with n uniform in
-// [32, 512] (buckets n <= 96, 192, 320, 512; every bucket chose the widest
-// band, the largest blocks and the most columns per sweep)
+// [32, 512] (buckets n <= 96, 192, 320, 512: every bucket chose the widest
+// band, the largest blocks and the most columns per sweep) and timings of
+// 512 pencils of one order each: for n <= 64 the one-stage reduction (r = 1)
+// takes 2.9 ms at n = 64 and 0.5 ms at n = 8, the best two-stage parameters
+// 11.4 ms and 1.1 ms
 void batch_bucket(int n, int& r, int& p, int& nq) {
-  (void)n;
+  if (n <= 64) {
+    r = 1;
+    p = 2;
+    nq = 1;
+    return;
+  }
   r = 16;
   p = 8;
   nq = 16;

@@ -7,6 +7,7 @@ module paraht_capi
 
    integer(c_int), parameter :: PARAHT_TIME_COMPONENTS = 1
    integer(c_int), parameter :: PARAHT_STAGE2_GIVENS = 2
+   integer(c_int), parameter :: PARAHT_QZ_SCHUR = 8
 
    interface
       integer(c_int) function paraht_ab2rht(n, A, lda, B, ldb, Q, ldq, Z, ldz, nb, p, flags) &
@@ -35,6 +36,16 @@ module paraht_capi
          import :: c_int, c_double
          integer(c_int), value :: n, lda, ldb, ldq, nb, p
          real(c_double) :: A(*), B(*), Q(*)
+      end function
+
+      ! QZ iteration on the reduced pencil (DHGEQZ's place after the HT
+      ! form); flags: PARAHT_QZ_SCHUR for the Schur form with Q, Z
+      integer(c_int) function paraht_hgeqz(n, H, ldh, T, ldt, Q, ldq, Z, ldz, alphar, alphai, &
+            beta, flags, info) bind(C, name='paraht_hgeqz')
+         import :: c_int, c_double
+         integer(c_int), value :: n, ldh, ldt, ldq, ldz, flags
+         real(c_double) :: H(*), T(*), Q(*), Z(*), alphar(*), alphai(*), beta(*)
+         integer(c_int) :: info
       end function
 
       integer(c_int) function paraht_version() bind(C, name='paraht_version')

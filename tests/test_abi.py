@@ -103,3 +103,12 @@ def test_qz_argument_errors_are_reported_without_a_device():
     # n = 0 is a quick return with info = 0
     assert L.paraht_hgeqz(0, None, 1, None, 1, None, 1, None, 1, None, None, None, 0,
                           ctypes.byref(info)) == 0 and info.value == 0
+
+
+def test_batched_size_buckets():
+    """The size-bucket table: the one-stage path (r = 1) up to order 64, the
+    two-stage parameters above (no device needed)."""
+    from paraht_amd.batched import ONE_STAGE_MAX, defaults
+    assert defaults(1)[0] == 1 and defaults(ONE_STAGE_MAX)[0] == 1
+    assert defaults(ONE_STAGE_MAX + 1) == (16, 8, 16)
+    assert defaults(512) == (16, 8, 16)

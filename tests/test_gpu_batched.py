@@ -126,3 +126,53 @@ def test_batched_graph_replay_is_exact():
         reduce_batch(bt, desc=desc, graph=True)
         for X, Y in zip((bt.A, bt.B, bt.Q, bt.Z), ref):
             assert torch.equal(X, Y), rep
+
+
+def test_one_stage_tiny_pencils_match_dgghrd(oracle_mod):
+    """r = 1: pencils of order <= 64 reduced in one stage with DGGHRD's
+    Givens sequence, in the same launch as two-stage pencils.  Each matches
+    LAPACK DGGHRD (the oracle's binding) elementwise, without sign
+    normalisation, within 1e-12 (relative, Frobenius), and keeps the
+    invariants; a two-stage pencil beside them still matches its oracle."""
+    from paraht_amd.batched import Batch, reduce_batch
+    from paraht_amd.pencils import illcond_pencil, random_pencil
+    O = oracle_mod
+    sizes = [1, 2, 3, 4, 5, 8, 17, 31, 33, 50, 63, 64, 100]
+    print(f"one-stage sizes={sizes}", flush=True)
+    pens = [(illcond_pencil if i % 3 == 2 else random_pencil)(n, seed=800 + i)
+            for i, n in enumerate(sizes)]
+    bt = Batch(sizes)
+    bt.load([a for a, _ in pens], [b for _, b in pens])
+    desc = bt.descriptors()
+    for i, n in enumerate(sizes):
+        if n <= 64:
+            desc[i].r, desc[i].p, desc[i].nq = 1, 0, 0
+        else:
+            desc[i].r, desc[i].p, desc[i].nq = 16, 4, 8
+    reduce_batch(bt, desc=desc)
+    for i, n in enumerate(sizes):
+        A0, B0 = pens[i]
+        H, T, Q, Z = _host(bt, i)
+        if n > 2:
+            ea, eb = O.backward_errors(A0, B0, H, T, Q, Z)
+            assert ea <= 10 * n * U and eb <= 10 * n * U, (n, ea, eb)
+        assert O.below_band(H, 1) == 0.0 and O.below_band(T, 0) == 0.0, n
+        if n <= 64:
+            ref = O.dgghrd(A0, B0)
+        else:
+            ref = O.sign_normalize(*O.reduce(A0, B0, 16, 4, 8, stage2="givens"))
+            H, T, Q, Z = O.sign_normalize(H, T, Q, Z)
+        scale = [np.linalg.norm(A0) or 1.0, np.linalg.norm(B0) or 1.0, np.sqrt(n), np.sqrt(n)]
+        tol = 1e-12 if n <= 64 else 1e-9
+        for X, Y, sc in zip((H, T, Q, Z), ref, scale):
+            assert _rel(X, Y, sc) <= tol, (n, _rel(X, Y, sc))
+
+
+def test_one_stage_refuses_larger_pencils():
+    from paraht_amd import ParahtError
+    from paraht_amd.batched import Batch, reduce_batch
+    bt = Batch([65])
+    desc = bt.descriptors()
+    desc[0].r, desc[0].p, desc[0].nq = 1, 0, 0
+    with pytest.raises(ParahtError):
+        reduce_batch(bt, desc=desc)

@@ -1,6 +1,6 @@
 """GPU QZ iteration (paraht_hgeqz*, paraht_amd/csrc/qz.hip) against LAPACK
-(scipy.linalg.eigvals, DGGEV) and the numpy statement of the same algorithm
-(tests/qz_emul.py):
+(scipy.linalg.eigvals, i.e. DGGEV, and DHGEQZ itself through the oracle's
+binding) and the numpy statement of the same algorithm (tests/qz_emul.py):
   * eigenvalues in the chordal metric within 1e-11 (LAPACK and the numpy
     statement agree with each other to ~1e-14 on these pencils), infinite
     eigenvalues of singular B included;
@@ -46,6 +46,9 @@ def test_qz_eigenvalues_match_lapack(oracle_mod, gen, n):
     assert E.match_chordal(ar + 1j * ai, be, ref[0], ref[1]) <= TOL
     ear, eai, ebe, _ = E.hgeqz(H.copy(), T.copy())
     assert E.match_chordal(ar + 1j * ai, be, ear + 1j * eai, ebe) <= TOL
+    *_, lar, lai, lbe, linfo = oracle_mod.dhgeqz(H, T)
+    assert linfo == 0
+    assert E.match_chordal(ar + 1j * ai, be, lar + 1j * lai, lbe) <= TOL
     assert (be >= 0).all()
 
 

@@ -157,3 +157,15 @@ def test_planted_fill_row_skip_fails(oracle_mod, monkeypatch):
     ar, ai, be, info = E.hgeqz(H.copy(), T.copy(), Q.copy(), Z.copy(), schur=True)
     ref = sl.eigvals(A0, B0, homogeneous_eigvals=True)
     assert info != 0 or E.match_chordal(ar + 1j * ai, be, ref[0], ref[1]) > 1e-6
+
+
+@pytest.mark.parametrize("n", [6, 45, 130])
+def test_lapack_dhgeqz_agrees(oracle_mod, n):
+    """The LAPACK routine the GPU iteration replaces (DHGEQZ, through the
+    oracle's binding) gives the same eigenvalues as the numpy statement."""
+    A0, B0 = illcond_pencil(n, seed=n)
+    H, T, Q, Z = ht_pencil(oracle_mod, A0, B0)
+    *_, lar, lai, lbe, linfo = oracle_mod.dhgeqz(H, T)
+    ar, ai, be, info = E.hgeqz(H.copy(), T.copy())
+    assert linfo == 0 and info == 0
+    assert E.match_chordal(ar + 1j * ai, be, lar + 1j * lai, lbe) <= 1e-12
