@@ -379,6 +379,8 @@ def main():
                     help="skip the forward + backward measurement")
     ap.add_argument("--no-qz", action="store_true",
                     help="skip the QZ iteration measurements")
+    ap.add_argument("--qz-only", action="store_true",
+                    help="only the QZ iteration measurements (one JSON object)")
     ap.add_argument("--no-other", action="store_true",
                     help="skip the reduction with the other stage-2 variant")
     args = ap.parse_args()
@@ -386,6 +388,9 @@ def main():
         print("bench.py runs on one GPU only: the HT reduction of one pencil is "
               "not sharded across GPUs in this build (--gpus 1).", file=sys.stderr)
         sys.exit(2)
+    if args.qz_only:
+        print(json.dumps({"qz": bench_qz(args)}), flush=True)
+        return
 
     import torch
 

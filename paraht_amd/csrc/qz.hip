@@ -14,7 +14,12 @@
 // left reflector, per row for a right one) between barriers.  In a bulge
 // sweep, thread 0 also applies the step's three reflectors to the corner the
 // next step reads (rows k..k+3, columns k-1..k+2 of H; rows and columns
-// k..k+2 of T), so each step needs one serial phase and two barriers.
+// k..k+2 of T), so each step needs one serial phase and two barriers.  The
+// corner stays in LDS from step to step; its new column comes from the
+// far-column update thread 0 takes itself and from entries no step has
+// touched (prefetched a step ahead), and its stores go out after the first
+// barrier, beside the other threads' work, so the serial phase waits on no
+// global memory.
 #include "device_util.hpp"
 #include "kernels.hpp"
 
@@ -30,25 +35,44 @@ struct QzRefl {
   int nv;
 };
 
-// v = (1, v1, v2), tau, beta with P x = beta e1 (x2 unused for nv = 2)
+// v = (1, v1, v2), tau, beta with P x = beta e1 (x2 unused for nv = 2):
+// beta = -sign(x0) ||x||, tau = (beta - x0) / beta, v = x / (x0 - beta).
+// Without division or square root: x is scaled by the power of two that
+// brings its largest magnitude into [0.5, 1) (exact), 1/||x|| comes from
+// v_rsq_f64 and 1/(|x0| + ||x||) from v_rcp_f64, each refined by two Newton
+// steps (relative error below 2^-52), and tau = 1 + |x0| / ||x||.
 __device__ inline QzRefl qz_house(double x0, double x1, double x2, int nv, double* beta) {
   QzRefl r;
   r.nv = nv;
   r.v0 = 1.0;
   r.v1 = 0.0;
   r.v2 = 0.0;
-  const double xn = nv == 3 ? hypot(x1, x2) : fabs(x1);
-  if (xn == 0.0) {
+  if (nv != 3) x2 = 0.0;
+  if (x1 == 0.0 && x2 == 0.0) {
     r.tau = 0.0;
     if (beta) *beta = x0;
     return r;
   }
-  const double b = -copysign(hypot(x0, xn), x0);
-  r.tau = (b - x0) / b;
-  const double sc = 1.0 / (x0 - b);
-  r.v1 = x1 * sc;
-  r.v2 = nv == 3 ? x2 * sc : 0.0;
-  if (beta) *beta = b;
+  const double mx = fmax(fabs(x0), fmax(fabs(x1), fabs(x2)));
+  const int e = __builtin_amdgcn_frexp_exp(mx);
+  const double s0 = ldexp(x0, -e), s1 = ldexp(x1, -e), s2 = ldexp(x2, -e);
+  const double d2 = fma(s0, s0, fma(s1, s1, s2 * s2));  // in [0.25, 3]
+  double y = __builtin_amdgcn_rsq(d2);
+  double q = fma(-d2 * y, y, 1.0);
+  y = fma(0.5 * y, q, y);
+  q = fma(-d2 * y, y, 1.0);
+  y = fma(0.5 * y, q, y);
+  const double d = d2 * y;  // ||x|| scaled
+  const double a0 = fabs(s0);
+  const double den = a0 + d;  // |x0 - beta| scaled, in [0.5, 3.5]
+  double w = __builtin_amdgcn_rcp(den);
+  w = fma(w, fma(-den, w, 1.0), w);
+  w = fma(w, fma(-den, w, 1.0), w);
+  const double sc = x0 < 0.0 ? -w : w;  // 1 / (x0 - beta), scaled
+  r.tau = fma(a0, y, 1.0);
+  r.v1 = s1 * sc;
+  r.v2 = s2 * sc;
+  if (beta) *beta = ldexp(x0 < 0.0 ? d : -d, e);
   return r;
 }
 
@@ -104,6 +128,11 @@ struct QzMats {
   __device__ double& t(int i, int j) const { return T[PHT_IDX(i, j, ldt)]; }
 };
 
+struct QzCorner {
+  double h[4][4];  // h[i][j] = H(k+i, k-1+j)
+  double t[3][3];  // t[i][j] = T(k+i, k+j)
+};
+
 struct QzShared {
   QzRefl L, R1, R2;
   double x0, x1, x2;  // first column of the shift polynomial
@@ -111,6 +140,8 @@ struct QzShared {
   int ired[2];
   int act;
   int flipa, flipb;
+  QzCorner corner;    // thread 0's corner between bulge steps
+  double next[7];     // H(k+1..k+2, k+3), T(k+1..k+2, k+3), H(k+3, k+3), H(k+4, k+3), T(k+3, k+3)
 };
 
 // left reflector on rows r0.. of H (columns hc0..last), T (tc0..last) and the
@@ -149,88 +180,126 @@ __device__ void qz_right(const QzMats& M, int c0, const QzRefl& R, int first, in
 // ---- one bulge step: the corner on thread 0 ------------------------------
 // Rows k..k+nr-1 get the left reflector L, columns k..k+nr-1 the right
 // reflectors R1 (nr = 3: zeroes T(k+2, k:k+1)) and R2 (zeroes T(k+1, k)).
-__device__ void qz_corner(const QzMats& M, QzShared& sh, int k, int lo, int m) {
-  const int nr = k <= m - 2 ? 3 : 2;
-  const bool top = k == lo;
-  const int hrows = min(3, m - k);  // last corner row of H (k + hrows)
-  double h[4][4];                    // h[i][j] = H(k+i, k-1+j)
-  double t[3][3];                    // t[i][j] = T(k+i, k+j)
+// The corner is H(k..k+3, k-1..k+2) and T(k..k+2, k..k+2); thread 0 keeps it
+// in LDS from one step to the next (entries outside the block [lo, m] or
+// below the first step's column are 0).
+
+// which corner entries exist at step k: rows k..k+2 from column k-1 (k-1 >= lo
+// only below the top) to min(k+2, m); row k+3 (if <= m) from column k
+__device__ __forceinline__ bool qz_hin(int i, int j, int k, int lo, int m) {
+  return k + i <= m && (i < 3 ? (j > 0 || k > lo) && k - 1 + j <= m : j > 0);
+}
+
+// the first step of a sweep: the corner from memory (row k+3 holds only
+// H(k+3, k+2) before the step)
+__device__ __forceinline__ void qz_corner_load(const QzMats& M, QzCorner& C, int k, int lo,
+                                               int m) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      // rows k..k+2: columns k-1 (below the top) .. min(k+2, m); row k+3:
-      // only H(k+3, k+2) is nonzero before the step
-      const bool in = i <= hrows && (i < 3 ? (j > 0 || !top) && k - 1 + j <= m : j == 3);
-      h[i][j] = in ? M.h(k + i, k - 1 + j) : 0.0;
-    }
+    for (int j = 0; j < 4; ++j)
+      C.h[i][j] = (qz_hin(i, j, k, lo, m) && (i < 3 || j == 3)) ? M.h(k + i, k - 1 + j) : 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) t[i][j] = (i < nr && j < nr) ? M.t(k + i, k + j) : 0.0;
+    for (int j = 0; j < 3; ++j) C.t[i][j] = (k + i <= m && k + j <= m) ? M.t(k + i, k + j) : 0.0;
+}
 
-  // left reflector
+// step k's corner from step k-1's: the overlap moves up and left; column
+// k+2 comes from the far-column update of step k-1 (rows k, k+1: hk, tk) and
+// from entries no step has touched yet (H(k+2, k+2), H(k+3, k+2), T(k+2, k+2))
+__device__ __forceinline__ void qz_corner_shift(QzCorner& C, int k, int m, double hk0, double hk1,
+                                                double tk0, double tk1, double h22, double h32,
+                                                double t22) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C.h[i][j] = C.h[i + 1][j + 1];
+  const bool c2 = k + 2 <= m, r3 = k + 3 <= m;
+  C.h[0][3] = c2 ? hk0 : 0.0;
+  C.h[1][3] = c2 ? hk1 : 0.0;
+  C.h[2][3] = c2 ? h22 : 0.0;
+  C.h[3][0] = 0.0;
+  C.h[3][1] = 0.0;
+  C.h[3][2] = 0.0;
+  C.h[3][3] = r3 ? h32 : 0.0;
+  C.t[0][0] = C.t[1][1];
+  C.t[0][1] = C.t[1][2];
+  C.t[1][0] = C.t[2][1];
+  C.t[1][1] = C.t[2][2];
+  C.t[0][2] = c2 ? tk0 : 0.0;
+  C.t[1][2] = c2 ? tk1 : 0.0;
+  C.t[2][2] = c2 ? t22 : 0.0;
+  C.t[2][0] = 0.0;
+  C.t[2][1] = 0.0;
+}
+
+// the step's reflectors, applied to the corner; published in sh
+__device__ __forceinline__ void qz_corner_step(QzCorner& C, QzShared& sh, int k, int lo, int m) {
+  const int nr = k <= m - 2 ? 3 : 2;
+  const bool top = k == lo;
   double beta = 0.0;
   QzRefl L;
   if (top)
     L = qz_house(sh.x0, sh.x1, sh.x2, nr, &beta);
   else
-    L = qz_house(h[0][0], h[1][0], h[2][0], nr, &beta);
+    L = qz_house(C.h[0][0], C.h[1][0], C.h[2][0], nr, &beta);
   if (!top) {
-    h[0][0] = beta;
-    h[1][0] = 0.0;
-    h[2][0] = 0.0;
+    C.h[0][0] = beta;
+    C.h[1][0] = 0.0;
+    C.h[2][0] = 0.0;
   }
 #pragma unroll
-  for (int j = 1; j < 4; ++j) qz_apply3(L, h[0][j], h[1][j], h[2][j]);
+  for (int j = 1; j < 4; ++j) qz_apply3(L, C.h[0][j], C.h[1][j], C.h[2][j]);
 #pragma unroll
-  for (int j = 0; j < 3; ++j) qz_apply3(L, t[0][j], t[1][j], t[2][j]);
-  // right reflectors on the corner rows
+  for (int j = 0; j < 3; ++j) qz_apply3(L, C.t[0][j], C.t[1][j], C.t[2][j]);
   QzRefl R1;
   R1.tau = 0.0;
   R1.nv = 3;
   R1.v0 = R1.v1 = R1.v2 = 0.0;
   if (nr == 3) {
-    R1 = qz_house_rev(t[2][0], t[2][1], t[2][2], 3);
+    R1 = qz_house_rev(C.t[2][0], C.t[2][1], C.t[2][2], 3);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) qz_apply3(R1, h[i][1], h[i][2], h[i][3]);
+    for (int i = 0; i < 4; ++i) qz_apply3(R1, C.h[i][1], C.h[i][2], C.h[i][3]);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) qz_apply3(R1, t[i][0], t[i][1], t[i][2]);
-    t[2][0] = 0.0;
-    t[2][1] = 0.0;
+    for (int i = 0; i < 3; ++i) qz_apply3(R1, C.t[i][0], C.t[i][1], C.t[i][2]);
+    C.t[2][0] = 0.0;
+    C.t[2][1] = 0.0;
   }
-  QzRefl R2 = qz_house_rev(t[1][0], t[1][1], 0.0, 2);
+  QzRefl R2 = qz_house_rev(C.t[1][0], C.t[1][1], 0.0, 2);
   R2.v2 = 0.0;
-  {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
     double z = 0.0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      z = 0.0;
-      qz_apply3(R2, h[i][1], h[i][2], z);
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      z = 0.0;
-      qz_apply3(R2, t[i][0], t[i][1], z);
-    }
+    qz_apply3(R2, C.h[i][1], C.h[i][2], z);
   }
-  t[1][0] = 0.0;
-  // store what exists
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double z = 0.0;
+    qz_apply3(R2, C.t[i][0], C.t[i][1], z);
+  }
+  C.t[1][0] = 0.0;
+  if (k + 3 > m) {  // keep entries outside the block at 0 for the next shift
+#pragma unroll
+    for (int j = 0; j < 4; ++j) C.h[3][j] = 0.0;
+  }
+  sh.L = L;
+  sh.R1 = R1;
+  sh.R2 = R2;
+}
+
+__device__ __forceinline__ void qz_corner_store(const QzMats& M, const QzCorner& C, int k, int lo,
+                                                int m) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const bool in = i <= hrows && (i < 3 ? (j > 0 || !top) && k - 1 + j <= m : j > 0);
-      if (in) M.h(k + i, k - 1 + j) = h[i][j];
-    }
+    for (int j = 0; j < 4; ++j)
+      if (qz_hin(i, j, k, lo, m)) M.h(k + i, k - 1 + j) = C.h[i][j];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-      if (i < nr && j < nr) M.t(k + i, k + j) = t[i][j];
-  sh.L = L;
-  sh.R1 = R1;
-  sh.R2 = R2;
+      if (k + i <= m && k + j <= m) M.t(k + i, k + j) = C.t[i][j];
 }
 
 // ---- the kernel ----------------------------------------------------------
@@ -531,41 +600,85 @@ __global__ __launch_bounds__(1024) void k_qz(const QzTask* __restrict__ tasks, d
       sh.x1 = h21 * w1 + h22 * w2 - s * y2;
       sh.x2 = h32 * w2;
     }
+    // Per step: thread 0 builds the corner (from the previous step's corner,
+    // the far column it updated itself and prefetched entries), barrier, then
+    // every thread takes far columns (H and T together: thread 0 takes
+    // column k+nr, which the next corner needs), rows above the corner, and
+    // rows of Q and Z, while thread 0 also stores the corner; barrier.
     for (int k = lo; k < m; ++k) {
-      if (tid == 0) qz_corner(M, sh, k, lo, m);
+      if (tid == 0) {
+        QzCorner C;
+        if (k == lo) {
+          qz_corner_load(M, C, k, lo, m);
+        } else {
+          C = sh.corner;
+          qz_corner_shift(C, k, m, sh.next[0], sh.next[1], sh.next[2], sh.next[3], sh.next[4],
+                          sh.next[5], sh.next[6]);
+        }
+        qz_corner_step(C, sh, k, lo, m);
+        sh.corner = C;
+      }
       __syncthreads();
       const QzRefl L = sh.L, R1 = sh.R1, R2 = sh.R2;
       const int nr = L.nv;
-      // far columns (left), rows above the corner (right), Q, Z
+      // untouched until the next step, which needs them: loaded now, kept in
+      // LDS after thread 0's own updates (their latencies overlap)
+      double pre0 = 0.0, pre1 = 0.0, pre2 = 0.0;
+      if (tid == 0) {
+        pre0 = k + 3 <= m ? M.h(k + 3, k + 3) : 0.0;
+        pre1 = k + 4 <= m ? M.h(k + 4, k + 3) : 0.0;
+        pre2 = k + 3 <= m ? M.t(k + 3, k + 3) : 0.0;
+        qz_corner_store(M, sh.corner, k, lo, m);
+      }
       const int c0 = k + nr;
       const int nc = max(last - c0 + 1, 0);
       const int nrw = max(k - first, 0);
       const int nqr = M.vq ? n : 0, nzr = M.vz ? n : 0;
-      const int tot = 2 * nc + 2 * nrw + nqr + nzr;
+      const int tot = nc + nrw + nqr + nzr;
       for (int x = tid; x < tot; x += blockDim.x) {
-        if (x < 2 * nc) {
-          double* p = x < nc ? &M.h(k, c0 + x) : &M.t(k, c0 + x - nc);
-          qz_apply(L, p, 1);
-        } else if (x < 2 * nc + 2 * nrw) {
-          const int y = x - 2 * nc;
-          double* p;
-          size_t ld;
-          if (y < nrw) {
-            p = &M.h(first + y, k);
-            ld = M.ldh;
-          } else {
-            p = &M.t(first + y - nrw, k);
-            ld = M.ldt;
+        if (x < nc) {
+          double* ph = &M.h(k, c0 + x);
+          double* pt = &M.t(k, c0 + x);
+          double a0 = ph[0], a1 = ph[1], a2 = nr == 3 ? ph[2] : 0.0;
+          double b0 = pt[0], b1 = pt[1], b2 = nr == 3 ? pt[2] : 0.0;
+          qz_apply3(L, a0, a1, a2);
+          qz_apply3(L, b0, b1, b2);
+          ph[0] = a0;
+          ph[1] = a1;
+          pt[0] = b0;
+          pt[1] = b1;
+          if (nr == 3) {
+            ph[2] = a2;
+            pt[2] = b2;
           }
-          if (nr == 3) qz_apply(R1, p, ld);
-          qz_apply(R2, p, ld);
-        } else if (x < 2 * nc + 2 * nrw + nqr) {
-          qz_apply(L, &M.Q[PHT_IDX(x - 2 * nc - 2 * nrw, k, M.ldq)], M.ldq);
+          if (x == 0) {  // rows k+1, k+2 of column k+3 for the next corner
+            sh.next[0] = a1;
+            sh.next[1] = a2;
+            sh.next[2] = b1;
+            sh.next[3] = b2;
+          }
+        } else if (x < nc + nrw) {
+          const int i = first + x - nc;
+          double* ph = &M.h(i, k);
+          double* pt = &M.t(i, k);
+          if (nr == 3) {
+            qz_apply(R1, ph, M.ldh);
+            qz_apply(R1, pt, M.ldt);
+          }
+          qz_apply(R2, ph, M.ldh);
+          qz_apply(R2, pt, M.ldt);
+        } else if (x < nc + nrw + nqr) {
+          qz_apply(L, &M.Q[PHT_IDX(x - nc - nrw, k, M.ldq)], M.ldq);
         } else {
-          double* p = &M.Z[PHT_IDX(x - 2 * nc - 2 * nrw - nqr, k, M.ldz)];
+          double* p = &M.Z[PHT_IDX(x - nc - nrw - nqr, k, M.ldz)];
           if (nr == 3) qz_apply(R1, p, M.ldz);
           qz_apply(R2, p, M.ldz);
         }
+      }
+      if (tid == 0) {
+        sh.next[4] = pre0;
+        sh.next[5] = pre1;
+        sh.next[6] = pre2;
       }
       __syncthreads();
     }
