@@ -269,7 +269,7 @@ def bench_qz(args):
     """The QZ iteration after the reduction (paraht_hgeqz*): generalized
     eigenvalues of (1) single pencils of order 1000 and 2000 (A ~ N(0,1),
     B = R of the QR of an N(0,1) matrix, reduced on the GPU first, untimed),
-    eigenvalues only and the Schur form with Q, Z; (2) the batched draw of
+    eigenvalues only and, at 1000, the Schur form with Q, Z; (2) the batched draw of
     2048 pencils with n uniform in [32, 512] after their batched reduction,
     eigenvalues only, in one launch.  CPU baseline: LAPACK DHGEQZ (scipy's
     OpenBLAS) on the order-1000 pencil and on every 64th pencil of the
@@ -292,7 +292,7 @@ def bench_qz(args):
         H, T, Q, Z = ph.reduce(A, B, 16, 4, 8, givens=True)
         host[n] = (H.cpu().numpy(), T.cpu().numpy())
         rec = {"n": n}
-        for mode in ("eigenvalues", "schur_qz"):
+        for mode in ("eigenvalues", "schur_qz") if n <= 1000 else ("eigenvalues",):
             h, t = ph.column_major(H.clone()), ph.column_major(T.clone())
             q, z = ph.column_major(Q.clone()), ph.column_major(Z.clone())
             torch.cuda.synchronize()
@@ -304,8 +304,8 @@ def bench_qz(args):
             torch.cuda.synchronize()
             rec[mode + "_s"] = time.perf_counter() - t0
             rec[mode + "_info"] = info
-        print(f"qz: n={n} eigenvalues {rec['eigenvalues_s']:.3f} s, Schur+Q,Z "
-              f"{rec['schur_qz_s']:.3f} s", file=sys.stderr, flush=True)
+        print(f"qz: n={n} " + ", ".join(f"{k[:-2]} {v:.3f} s" for k, v in rec.items()
+                                         if k.endswith("_s")), file=sys.stderr, flush=True)
         out["single"].append(rec)
     rng = np.random.default_rng(1302)
     sizes = [int(x) for x in rng.integers(32, 513, size=2048)]
