@@ -95,7 +95,9 @@ def _forward(A, B, r, p, nq):
             r_, p_, nq_ = defaults(A.shape[0]) if A.shape[0] <= 512 else (64, 2, 16)
             if r_ == 1:  # the one-stage path exists only in the batched kernel
                 r_, p_, nq_ = 8, 2, 4
-        H, T, Q, Z = ph.reduce(H, T, r_, p_, nq_)
+        # the adjoint needs only (H, T, Q, Z), not the variant that produced
+        # them: the faster Givens chase
+        H, T, Q, Z = ph.reduce(H, T, r_, p_, nq_, givens=True)
         return H, T, Q, Z
     bsz, n, _ = A.shape
     bt = Batch([n] * bsz, device=A.device)
