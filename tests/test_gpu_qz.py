@@ -143,3 +143,29 @@ def test_qz_order_1000_eigenvalues(oracle_mod):
     assert info == 0
     ref = sl.eigvals(H, T, homogeneous_eigvals=True)
     assert E.match_chordal(ar + 1j * ai, be, ref[0], ref[1]) <= TOL
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 24, 300])
+def test_qz_edge_pencils(n):
+    """B = 0 (every eigenvalue infinite), H = 0 (every eigenvalue zero), an
+    already triangular pencil (its diagonal pairs), a zero subdiagonal in the
+    middle; and a NaN entry, which returns info = n at once."""
+    from test_qz_emul import _edge_pencils
+    rng = np.random.default_rng(n)
+    print(f"qz edge pencils n={n}", flush=True)
+    for name, H, T, known in _edge_pencils(n, rng):
+        for schur in (False, True):
+            *_, ar, ai, be, info = _gpu(H, T, np.eye(n), np.eye(n), schur=schur)
+            assert info == 0, name
+            if name == "B=0":
+                assert (be == 0).all(), name
+            elif name == "H=0":
+                assert (ar == 0).all() and (ai == 0).all() and (be > 0).all(), name
+            else:
+                if known is None:
+                    known = sl.eigvals(H, T, homogeneous_eigvals=True)
+                assert E.match_chordal(ar + 1j * ai, be, known[0], known[1]) <= TOL, name
+    H = np.triu(rng.standard_normal((n, n)), -1)
+    H[n // 2, n // 2] = np.nan
+    *_, info = _gpu(H, np.eye(n))
+    assert info == n

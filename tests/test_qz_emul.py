@@ -169,3 +169,37 @@ def test_lapack_dhgeqz_agrees(oracle_mod, n):
     ar, ai, be, info = E.hgeqz(H.copy(), T.copy())
     assert linfo == 0 and info == 0
     assert E.match_chordal(ar + 1j * ai, be, lar + 1j * lai, lbe) <= 1e-12
+
+
+def _edge_pencils(n, rng):
+    """Edge cases of the QZ iteration with known eigenvalues (alpha, beta):
+    B = 0 (all infinite), H = 0 (all zero), an already triangular pencil
+    (the diagonal pairs), and a block-diagonal pencil with a zero
+    subdiagonal in the middle (two independent halves)."""
+    H = np.triu(rng.standard_normal((n, n)), -1)
+    T = np.triu(rng.standard_normal((n, n))) + 3 * np.eye(n)
+    cases = [("B=0", H.copy(), np.zeros((n, n)), None),
+             ("H=0", np.zeros((n, n)), T.copy(), None)]
+    Ht = np.triu(H)
+    cases.append(("triangular", Ht, T.copy(), (np.diag(Ht).astype(complex), np.diag(T))))
+    Hs = H.copy()
+    Hs[n // 2, n // 2 - 1] = 0.0
+    cases.append(("split", Hs, T.copy(), None))
+    return cases
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 24])
+def test_edge_pencils(n):
+    rng = np.random.default_rng(n)
+    for name, H, T, known in _edge_pencils(n, rng):
+        for schur in (False, True):
+            ar, ai, be, info = E.hgeqz(H.copy(), T.copy(), np.eye(n), np.eye(n), schur=schur)
+            assert info == 0, name
+            if name == "B=0":
+                assert (be == 0).all(), name
+            elif name == "H=0":
+                assert (ar == 0).all() and (ai == 0).all() and (be > 0).all(), name
+            else:
+                if known is None:
+                    known = sl.eigvals(H, T, homogeneous_eigvals=True)
+                assert E.match_chordal(ar + 1j * ai, be, known[0], known[1]) <= 1e-12, name
