@@ -1,6 +1,6 @@
 ! Benchmark driver with the command line and result line of the reference's
 ! src/profilereduction.f90:
-!   profilereduction n algorithm [r p nq [time_components]] [pencil=FILE] [out=FILE]
+!   profilereduction n algorithm [r p nq [time_components]] [pencil=FILE] [out=FILE] [qz=1]
 ! algorithm 1: ab2rht_simple + rht2ht_householder (modules ab2rht / rht2ht,
 ! here the GPU drop-ins); algorithm 2: LAPACK DGGHD3 on the host.  The pencil
 ! is A uniform on [0,1), B the R factor of a uniform [0,1) matrix, Q = Z = I,
@@ -10,7 +10,12 @@
 ! carried into A and Q (on the GPU, paraht_qr_b, for algorithm 1; LAPACK
 ! dgeqrf/dormqr/dorgqr for algorithm 2) and timed with the reduction, and the
 ! errors are those against the pencil in the file.  out=FILE writes the
-! reduced H, T, Q, Z.  Prints
+! reduced H, T, Q, Z.  qz=1 then runs the QZ iteration on the reduced pencil
+! (eigenvalues only: paraht_hgeqz on the GPU for algorithm 1, LAPACK DHGEQZ
+! for algorithm 2; the reference stops at the HT form) and prints a second
+! line with its time, DHGEQZ's info and the number of infinite eigenvalues,
+! for algorithm 1 also the largest chordal distance from a GPU eigenvalue to
+! the nearest of DHGEQZ's on the same pencil.  Prints
 !   n: ;runtime: ;error: ||Q^T A0 Z - H||/||A0|| ||Q^T B0 Z - T||/||B0||
 ! (and for algorithm 1 the two phase times and r, p, nq).
 !
@@ -21,11 +26,11 @@
 program profilereduction
    use ab2rht
    use rht2ht
-   use paraht_capi, only: paraht_qr_b, paraht_check
+   use paraht_capi, only: paraht_qr_b, paraht_hgeqz, paraht_check
    implicit none
 
    integer :: n, algorithm, r, p, nq, flagarg, info, nseed, k, npos, iarg
-   logical :: time_components
+   logical :: time_components, want_qz
    character(len=4096) :: pencil_file, out_file, full
    double precision, allocatable :: A(:, :), B(:, :), Q(:, :), Z(:, :), A0(:, :), B0(:, :), W(:, :)
    double precision, allocatable :: tau(:), work(:)
@@ -40,6 +45,7 @@ program profilereduction
    time_components = .false.
    pencil_file = ''
    out_file = ''
+   want_qz = .false.
    ! key=value options may follow the positional arguments
    npos = 0
    do iarg = 1, command_argument_count()
@@ -48,12 +54,14 @@ program profilereduction
          pencil_file = full(8:)
       else if (full(1:4) == 'out=') then
          out_file = full(5:)
+      else if (full(1:3) == 'qz=') then
+         want_qz = full(4:4) == '1'
       else
          npos = npos + 1
       end if
    end do
    if (npos < 2) then
-      write (*, *) 'usage: profilereduction n algorithm (r p nq time_components) (pencil=FILE) (out=FILE)'
+      write (*, *) 'usage: profilereduction n algorithm (r p nq time_components) (pencil=FILE) (out=FILE) (qz=1)'
       stop
    end if
    call get_command_argument(1, arg); read (arg, *) n
@@ -144,8 +152,57 @@ program profilereduction
    else
       write (*, '("n:",I6," ;runtime:",F16.6," ;error:",2ES16.6)') n, total, ea, eb
    end if
+   if (want_qz) call run_qz()
 
 contains
+
+   ! The QZ iteration on the reduced pencil (A, B now hold H, T).
+   subroutine run_qz()
+      double precision, allocatable :: H(:, :), T(:, :), ar(:), ai(:), be(:)
+      double precision, allocatable :: lr(:), li(:), lb(:), wk(:)
+      double precision :: dq(1, 1), tq, dmax, dmin
+      integer :: st, qinfo, linfo, i, j, ninf
+      allocate (H(n, n), T(n, n), ar(n), ai(n), be(n), wk(max(1, n)))
+      H = A
+      T = B
+      call system_clock(c0)
+      if (algorithm == 1) then
+         st = paraht_hgeqz(n, H, n, T, n, dq, 1, dq, 1, ar, ai, be, 0, qinfo)
+         call paraht_check(st, 'paraht_hgeqz')
+      else
+         call dhgeqz('E', 'N', 'N', n, 1, n, H, n, T, n, ar, ai, be, dq, 1, dq, 1, wk, n, qinfo)
+      end if
+      call system_clock(c1, rate)
+      tq = dble(c1 - c0)/dble(rate)
+      ninf = count(be == 0.0d0)
+      if (algorithm == 1) then
+         allocate (lr(n), li(n), lb(n))
+         H = A
+         T = B
+         call dhgeqz('E', 'N', 'N', n, 1, n, H, n, T, n, lr, li, lb, dq, 1, dq, 1, wk, n, linfo)
+         dmax = 0.0d0
+         do i = 1, n
+            dmin = huge(1.0d0)
+            do j = 1, n
+               dmin = min(dmin, chordal(ar(i), ai(i), be(i), lr(j), li(j), lb(j)))
+            end do
+            dmax = max(dmax, dmin)
+         end do
+         write (*, '("qz: time:",F16.6," ;info:",I6," ;infinite:",I6," ;chordal vs dhgeqz:",ES16.6)') &
+            tq, qinfo, ninf, dmax
+      else
+         write (*, '("qz: time:",F16.6," ;info:",I6," ;infinite:",I6)') tq, qinfo, ninf
+      end if
+   end subroutine
+
+   ! chordal distance between (a1 + i c1)/b1 and (a2 + i c2)/b2
+   double precision function chordal(a1, c1, b1, a2, c2, b2)
+      double precision, intent(in) :: a1, c1, b1, a2, c2, b2
+      complex(8) :: x, y
+      x = cmplx(a1, c1, 8)
+      y = cmplx(a2, c2, 8)
+      chordal = abs(x*b2 - b1*y)/(sqrt(abs(x)**2 + b1**2)*sqrt(abs(y)**2 + b2**2))
+   end function
 
    ! The order of a pencil file (checked against a nonzero n from the
    ! command line).

@@ -10,8 +10,10 @@ leaves to LAPACK (it stops at the HT form, src/profilereduction.f90:152-162).
         every pencil of a paraht_amd.batched.Batch (after reduce_batch) in one
         launch; eigenvalues of pencil i at sum(sizes[:i]).
     eigvals(A, B, r, p, nq)  -> complex eigenvalues alpha/beta (inf where
-        beta = 0) of a pencil with upper-triangular B: both reduction stages,
-        then the QZ iteration, on the GPU.
+        beta = 0): the QR of a general B, both reduction stages, then the QZ
+        iteration, on the GPU.
+    eigvals_batch(As, Bs)  -> the same for many pencils (B triangular) in two
+        launches: the batched reduction and the batched QZ.
 """
 from __future__ import annotations
 
@@ -91,16 +93,46 @@ def hgeqz_batch(batch, schur=False, vectors=False, ctx=None):
     return ev[0], ev[1], ev[2], list(info)[:cnt]
 
 
+def _ratios(ar, ai, be):
+    alpha = np.asarray(ar) + 1j * np.asarray(ai)
+    be = np.asarray(be)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(be != 0.0, alpha / np.where(be != 0.0, be, 1.0), np.inf)
+
+
 def eigvals(A, B, r=16, p=4, nq=8, givens=False):
-    """Generalized eigenvalues of (A, B), B upper triangular: the two-stage
-    reduction and the QZ iteration on the GPU.  Complex array alpha/beta
-    (inf where beta = 0)."""
-    from . import reduce
-    H, T, _, _ = reduce(np.asarray(A, dtype=np.float64), np.asarray(B, dtype=np.float64),
-                        r, p, nq, givens=givens)
+    """Generalized eigenvalues of (A, B) on the GPU: the QR of B when B is
+    not upper triangular, both reduction stages, then the QZ iteration.
+    Complex array alpha/beta (inf where beta = 0)."""
+    from . import reduce, reduce_general
+    A = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    if np.tril(B, -1).any():
+        H, T, _, _ = reduce_general(A, B, r, p, nq, givens=givens)
+    else:
+        H, T, _, _ = reduce(A, B, r, p, nq, givens=givens)
     ar, ai, be, info = hgeqz(H, np.asfortranarray(T))
     if info:
         raise _lib.ParahtError(f"eigvals: QZ did not converge (info={info})")
-    alpha = ar + 1j * ai
-    with np.errstate(divide="ignore", invalid="ignore"):
-        return np.where(be != 0.0, alpha / np.where(be != 0.0, be, 1.0), np.inf)
+    return _ratios(ar, ai, be)
+
+
+def eigvals_batch(As, Bs, ctx=None):
+    """Generalized eigenvalues of many pencils (B upper triangular) on the
+    GPU: one batched reduction launch (size-bucket parameters) and one
+    batched QZ launch.  Returns a list of complex arrays alpha/beta."""
+    from .batched import Batch, reduce_batch
+    sizes = [int(a.shape[0]) for a in As]
+    bt = Batch(sizes)
+    bt.load(As, Bs)
+    reduce_batch(bt, ctx=ctx)
+    ar, ai, be, info = hgeqz_batch(bt, ctx=ctx)
+    bad = [i for i, x in enumerate(info) if x]
+    if bad:
+        raise _lib.ParahtError(f"eigvals_batch: QZ did not converge for pencils {bad[:8]}")
+    ar, ai, be = ar.cpu().numpy(), ai.cpu().numpy(), be.cpu().numpy()
+    out, off = [], 0
+    for n in sizes:
+        out.append(_ratios(ar[off:off + n], ai[off:off + n], be[off:off + n]))
+        off += n
+    return out

@@ -129,3 +129,17 @@ def test_driver_general_b_from_a_file(driver, tmp_path, oracle_mod):
     assert ea <= 10 * n * u and eb <= 10 * n * u, (ea, eb)
     assert O.below_band(H, 1) == 0.0 and O.below_band(T, 0) == 0.0
     assert O.chordal_eig_distance(A0, B0, H, T) <= 1e-6
+
+
+QZ_LINE = re.compile(r"qz: time:\s*([-\d.]+) ;info:\s*(\d+) ;infinite:\s*(\d+)")
+
+
+def test_driver_qz_lapack_path(driver):
+    """qz=1 after algorithm 2: LAPACK DHGEQZ on the reduced pencil, a second
+    result line with its time, info and the count of infinite eigenvalues."""
+    out = subprocess.run([driver, "150", "2", "qz=1"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert LINE.search(out.stdout)
+    m = QZ_LINE.search(out.stdout)
+    assert m and int(m.group(2)) == 0 and int(m.group(3)) == 0, out.stdout

@@ -86,3 +86,18 @@ def test_driver_paraht_general_b_from_a_file(tmp_path):
     assert ea <= 10 * n * u and eb <= 10 * n * u, (ea, eb)
     assert O.below_band(H, 1) == 0.0 and O.below_band(T, 0) == 0.0
     assert O.chordal_eig_distance(A0, B0, H, T) <= 1e-6
+
+
+def test_driver_qz_after_the_reduction():
+    """qz=1 after algorithm 1: the GPU QZ iteration (paraht_hgeqz through the
+    Fortran binding) on the reduced pencil; every eigenvalue lies within 1e-11
+    (chordal) of LAPACK DHGEQZ's on the same pencil."""
+    print("fortran driver qz=1 n=200", flush=True)
+    out = subprocess.run([dropin.DRIVER, "200", "1", "16", "4", "8", "qz=1"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert LINE.search(out.stdout), out.stdout
+    m = re.search(r"qz: time:\s*([-\d.]+) ;info:\s*(\d+) ;infinite:\s*(\d+) ;chordal vs "
+                  r"dhgeqz:\s*(\S+)", out.stdout)
+    assert m and int(m.group(2)) == 0, out.stdout
+    assert float(m.group(4)) <= 1e-11, out.stdout

@@ -169,3 +169,23 @@ def test_qz_edge_pencils(n):
     H[n // 2, n // 2] = np.nan
     *_, info = _gpu(H, np.eye(n))
     assert info == n
+
+
+def test_qz_eigvals_general_b_and_batch():
+    """paraht_amd.qz.eigvals on a pencil with a general B (the GPU QR of B
+    first) and eigvals_batch on pencils of mixed orders (one-stage and
+    two-stage reductions in one launch), against LAPACK on the original
+    pencils."""
+    rng = np.random.default_rng(21)
+    n = 150
+    A0 = rng.standard_normal((n, n))
+    B0 = rng.standard_normal((n, n))
+    ref = sl.eigvals(A0, B0)
+    lam = ph.qz.eigvals(A0, B0, 16, 4, 8)
+    assert E.match_chordal(lam, np.ones(n), ref, np.ones(n)) <= TOL
+    sizes = [5, 40, 64, 65, 130, 257]
+    pens = [random_pencil(k, seed=600 + k) for k in sizes]
+    got = ph.qz.eigvals_batch([a for a, _ in pens], [b for _, b in pens])
+    for (a, b), g in zip(pens, got):
+        ref = sl.eigvals(a, b)
+        assert E.match_chordal(g, np.ones(len(g)), ref, np.ones(len(g))) <= TOL
