@@ -688,7 +688,7 @@ __global__ __launch_bounds__(1024) void k_qz(const QzTask* __restrict__ tasks, d
 
 }  // namespace
 
-int qz_threads(int n) { return n >= 768 ? 1024 : (n >= 192 ? 512 : 256); }
+int qz_threads(int n) { return n >= 384 ? 1024 : (n >= 192 ? 512 : 256); }
 
 hipError_t qz_launch(hipStream_t st, const QzTask* dtasks, int count, int threads, double* H,
                      double* T, double* Q, double* Z, double* ar, double* ai, double* be,
