@@ -220,6 +220,20 @@ __device__ __noinline__ void far_updates_wg(double* sm, double* A, double* B, do
   __syncthreads();
 }
 
+// Stage 2 of one pencil.  The generation runs kGroups columns of a sweep at
+// once, one group of kBT / kGroups threads (whole waves) each: group g takes
+// the columns ic = g+1, g+1+kGroups, ..., and step (ic, ib) waits until column
+// ic-1 has finished step ib+2 (LDS progress counters), the order of the
+// single-pencil sweep whose leaders also take the rows above their window
+// (rht2ht.f90:787-957 in the reference's column order otherwise).  Each group
+// has its own LDS arena for the steps.
+constexpr int kGroups = 4;
+constexpr int kGT = kBT / kGroups;
+static_assert(kGT % 128 == 0, "a group holds whole pairs of waves");
+constexpr int kArena = kBRmax * (kBRmax + 1) + 4 * kBRmax + 2 * s2::kMaxLdr + 2 * (kGT / 128) * 128 +
+                       (2 * kBRmax) * (2 * kBRmax + 1);
+static_assert((kGroups * kArena + 64) * sizeof(double) <= s1::kRqLds, "stage-2 arenas fit");
+
 __device__ __noinline__ void stage2_wg(double* sm, double* A, double* B, double* Q, double* Z, int n, int r,
                           int nq, double* ws) {
   using namespace s2;
@@ -231,16 +245,22 @@ __device__ __noinline__ void stage2_wg(double* sm, double* A, double* B, double*
   double* CV = SU + tsz;
   double* SV = CV + tsz;
   double* Mw = SV + tsz;
-  // LDS arena of the chase steps
-  double(*Bb)[kBRmax + 1] = reinterpret_cast<double(*)[kBRmax + 1]>(sm);
-  double* lc = sm + kBRmax * (kBRmax + 1);
+  // this thread's group and its LDS arena
+  const int g = threadIdx.x / kGT;
+  double* arena = sm + (size_t)g * kArena;
+  double(*Bb)[kBRmax + 1] = reinterpret_cast<double(*)[kBRmax + 1]>(arena);
+  double* lc = arena + kBRmax * (kBRmax + 1);
   double* ls = lc + kBRmax;
   double* rc = ls + kBRmax;
   double* rs = rc + kBRmax;
   double* xa = rs + kBRmax;
   double* xb = xa + kMaxLdr;
   double* red = xb + kMaxLdr;
-  double* Ml = red + 2 * (kBT / 128) * 128;
+  double* Ml = red + 2 * (kGT / 128) * 128;
+  // column progress (index ic, 0 unused) and the groups' barrier counters
+  int* prog = reinterpret_cast<int*>(sm + (size_t)kGroups * kArena);
+  int* gcount = prog + 40;
+  constexpr int kDoneStep = 1 << 30;
   for (int s = 1; s <= n - 2; s += nq) {
     const int nblocks = (n - s) / r + ((n - s) % r != 0);
     const size_t tz = (size_t)ldr * nq * nblocks, msz = (size_t)ldr * ldr;
@@ -254,10 +274,11 @@ __device__ __noinline__ void stage2_wg(double* sm, double* A, double* B, double*
       const size_t k = e % msz;
       Mw[e] = ((k % ldr) == (k / ldr)) ? 1.0 : 0.0;
     }
+    for (int e = threadIdx.x; e < 40 + kGroups; e += kBT) prog[e] = 0;
     __threadfence_block();
     __syncthreads();
-    // generation in the reference's order (rht2ht.f90:787-957)
-    for (int ic = 1; ic <= nq; ++ic) {
+    WaveGroup grp{g * kGT, kGT / 64, &gcount[g], 0};
+    for (int ic = g + 1; ic <= nq; ic += kGroups) {
       const int jc = s + ic - 1;
       for (int ib = 1;; ++ib) {
         int col, row0;
@@ -270,12 +291,26 @@ __device__ __noinline__ void stage2_wg(double* sm, double* A, double* B, double*
           if (col > n) break;
           row0 = col + r;
         }
-        chase_step<kBRmax, kBT>(A, n, B, n, n, r, nq, s, ic, ib, col, row0, CU, SU, CV, SV, ldr,
-                                Mw, true, Bb, lc, ls, rc, rs, xa, xb, red, Ml);
-        __threadfence_block();
-        __syncthreads();
+        if (ic > 1) {  // column ic-1 is two steps ahead (or done); bounded wait
+          for (int spin = 0; __hip_atomic_load(&prog[ic - 1], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP) < ib + 2 &&
+                             spin < (1 << 26);
+               ++spin)
+            __builtin_amdgcn_s_sleep(1);
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+        chase_step<kBRmax, kGT>(grp, A, n, B, n, n, r, nq, s, ic, ib, col, row0, CU, SU, CV, SV,
+                                ldr, Mw, true, Bb, lc, ls, rc, rs, xa, xb, red, Ml);
+        grp.sync();  // the step's stores are done
+        if (grp.tid() == 0)
+          __hip_atomic_store(&prog[ic], ib, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
+      grp.sync();
+      if (grp.tid() == 0)
+        __hip_atomic_store(&prog[ic], kDoneStep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+    __threadfence_block();
+    __syncthreads();
     // far updates of the sweep
     bool done = false;
 #define PHT_FAR(RR, NN)                                                                      \

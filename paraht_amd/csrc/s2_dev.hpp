@@ -164,12 +164,13 @@ __device__ __forceinline__ void left_chain_lds(double* p, int st, int nb, const 
 }
 
 // M of one block (ldr x ldr, column-major, identity for the first column
-// ic = 1) into LDS (ld ldr+1) by nw waves starting at wave w0: a wave per set
+// ic = 1) into LDS (ld ldr+1) by nw waves starting at wave w0 (of the thread
+// group whose thread tid calls it): a wave per set
 // of columns, lanes along the rows (ldr <= kMaxLdr = 160 rows: three 64-row
 // chunks), 8 columns' loads in flight together.
 __device__ __forceinline__ void stage_m(double* Ml, const double* __restrict__ M, int ldr,
-                                        int ic, int w0, int nw) {
-  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) - w0;
+                                        int ic, int w0, int nw, int tid) {
+  const int lane = tid & 63, wave = (tid >> 6) - w0;
   const int ldl = ldr + 1;
   for (int cm0 = wave; cm0 < ldr; cm0 += 8 * nw) {
     double tmp[8][3];
@@ -196,8 +197,40 @@ __device__ __forceinline__ void stage_m(double* Ml, const double* __restrict__ M
   }
 }
 
+// The threads that run a chase step: a whole workgroup, or a group of whole
+// waves of it (the batched kernel runs several columns of a sweep at once,
+// one group each).  tid() is the thread's index in the group, sync() a
+// barrier of the group that orders its LDS and global memory accesses.
+struct WgThreads {
+  __device__ int tid() const { return threadIdx.x; }
+  __device__ void sync() { __syncthreads(); }
+};
+
+struct WaveGroup {
+  int base;     // first thread of the group (a multiple of 64)
+  int nwaves;   // waves in the group
+  int* count;   // the group's arrival counter in LDS (0 when the group starts)
+  int arrived;  // barriers this thread has passed
+  __device__ int tid() const { return (int)threadIdx.x - base; }
+  __device__ void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    ++arrived;
+    if ((threadIdx.x & 63) == 0)
+      __hip_atomic_fetch_add(count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const int goal = arrived * nwaves;
+    // bounded: the group's waves arrive within microseconds; a missing one
+    // ends the wait (and corrupts the result) instead of hanging the kernel
+    for (int spin = 0;
+         __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < goal &&
+         spin < (1 << 24);
+         ++spin)
+      __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+};
+
 // One step (ic, ib) of a sweep starting at column s (rht2ht.f90:787-957), by
-// all NT threads of a workgroup:
+// the NT threads of a group (grp: WgThreads or WaveGroup):
 //   * M of block ib (the product of the deferred left rotations of columns
 //     1..ic-1, identity for ic = 1) is staged in LDS (Ml, ld ldr+1) and
 //     applied to A(:, col) and B(:, row0+r-1) as one mat-vec;
@@ -212,8 +245,8 @@ __device__ __forceinline__ void stage_m(double* Ml, const double* __restrict__ M
 // buffer), waves 1.. stage M of block ib+1 into it while wave 0 runs the right
 // chain; the caller must have acquired that block's M (the previous column's
 // step ib+1 is complete).  Returns whether Mnext was filled.
-template <int RMAX, int NT>
-__device__ inline bool chase_step(double* __restrict__ A, int lda, double* __restrict__ B,
+template <int RMAX, int NT, class Grp>
+__device__ inline bool chase_step(Grp& grp, double* __restrict__ A, int lda, double* __restrict__ B,
                                   int ldb, int n, int r, int nq, int s, int ic, int ib, int col,
                                   int row0, double* __restrict__ CU, double* __restrict__ SU,
                                   double* __restrict__ CV, double* __restrict__ SV, int ldr,
@@ -222,7 +255,7 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
                                   double* xb, double* red, double* Ml, bool have_m = false,
                                   double* Mnext = nullptr) {
   constexpr int kParts = NT / 128;
-  const int tid = threadIdx.x;
+  const int tid = grp.tid();
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int ldl = ldr + 1;
@@ -233,8 +266,8 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
   double* M = Mw + (size_t)(ib - 1) * msz;
 
   // ---- M of this block into LDS (unless the previous step staged it)
-  if (!have_m) stage_m(Ml, M, ldr, ic, 0, NT / 64);
-  __syncthreads();
+  if (!have_m) stage_m(Ml, M, ldr, ic, 0, NT / 64, tid);
+  grp.sync();
   bool staged_next = false;
 
   // ---- deferred left rotations of the sweep's earlier columns:
@@ -247,7 +280,7 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
       xa[t] = A[PHT_IDX(base + t, col - 1, lda)];
       xb[t] = doB ? B[PHT_IDX(base + t, row0 + r - 2, ldb)] : 0.0;
     }
-    __syncthreads();
+    grp.sync();
     const int h = tid >> 7, il = tid & 127;
     const int jp = (L + kParts - 1) / kParts, j0 = h * jp, j1 = imin(L, j0 + jp);
     for (int i0 = 0; i0 < L; i0 += 128) {
@@ -262,7 +295,7 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
       }
       red[(0 * kParts + h) * 128 + il] = ya;
       red[(1 * kParts + h) * 128 + il] = yb;
-      __syncthreads();
+      grp.sync();
       if (h == 0 && i < L) {
         double sa = 0.0, sb = 0.0;
 #pragma unroll
@@ -273,7 +306,7 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
         A[PHT_IDX(base + i, col - 1, lda)] = sa;
         if (doB) B[PHT_IDX(base + i, row0 + r - 2, ldb)] = sb;
       }
-      __syncthreads();
+      grp.sync();
     }
   }
 
@@ -333,7 +366,7 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
         if (e < RMAX * RMAX) Bb[e % RMAX][e / RMAX] = tmp[q];
       }
     }
-    __syncthreads();
+    grp.sync();
 
     // ---- left rotations on the B block (rht2ht.f90:837-840,921-924):
     //      threads 0..nb-1, one block column each; meanwhile threads
@@ -342,7 +375,7 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
       left_chain_lds(&Bb[0][tid], RMAX + 1, nb, lc, ls);
     else if (tid >= 64 && tid < 64 + ldr)
       left_chain_lds(&Ml[(tid - 64) * ldl + (ic - 1)], 1, nb, lc, ls);
-    __syncthreads();
+    grp.sync();
     // the updated M rows go back while wave 0 runs the right chain
     if (wave > 0) {
       constexpr int PM = (RMAX * kMaxLdr + NT - 64 - 1) / (NT - 64);
@@ -352,14 +385,14 @@ __device__ inline bool chase_step(double* __restrict__ A, int lda, double* __res
         const int k = e % RMAX, cm = e / RMAX;
         if (k < nb && cm < ldr) M[(ic - 1) + k + (size_t)cm * ldr] = Ml[cm * ldl + (ic - 1) + k];
       }
-      if (NT > 64 && Mnext != nullptr) stage_m(Mnext, M + msz, ldr, ic, 1, NT / 64 - 1);
+      if (NT > 64 && Mnext != nullptr) stage_m(Mnext, M + msz, ldr, ic, 1, NT / 64 - 1, tid);
     }
     staged_next = NT > 64 && Mnext != nullptr;
 
     // ---- right chain on the B block (rht2ht.f90:843-851,927-935):
     //      wave 0, lane = block row; the pivot values are read from lane ig
     if (wave == 0) right_chain<RMAX>(Bb, nb, lane, rc, rs);
-    __syncthreads();
+    grp.sync();
 
     // ---- the B block and the tape go back, and the right rotations are
     //      applied to the window rows (rht2ht.f90:853-867,937-951): A rows

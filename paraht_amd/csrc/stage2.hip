@@ -146,6 +146,7 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
   const int jc = s + ic - 1;
   const size_t mstride = (size_t)ldr * (ldr + 1);
   bool have_m = false;
+  WgThreads wg;
 
   for (int ib = 1;; ++ib) {
     int col, row0;
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(kGenThreads) void k_s2_sweep(
     const bool next = jc + 1 + (ib - 1) * r <= n;
     double* Mcur = Ml + (mbufs == 2 ? (size_t)(ib & 1) * mstride : 0);
     double* Mnxt = (mbufs == 2 && next) ? Ml + (size_t)((ib + 1) & 1) * mstride : nullptr;
-    have_m = chase_step<RMAX, kGenThreads>(A, lda, B, ldb, n, r, nq, s, ic, ib, col, row0, CU,
+    have_m = chase_step<RMAX, kGenThreads>(wg, A, lda, B, ldb, n, r, nq, s, ic, ib, col, row0, CU,
                                            SU, CV, SV, ldr, Mw, nhelp == 0, Bb, lc, ls, rc, rs,
                                            xa, xb, &red[0][0][0], Mcur, have_m, Mnxt);
     publish(&progress[ic - 1], ib);
