@@ -116,104 +116,135 @@ __device__ __noinline__ void stage1_wg(double* sm, double* A, double* B, double*
 // rows of A, B above each block's window and all rows of Q, Z (right
 // rotations; Q takes the transposed ones), then columns of A, B right of
 // the window (left rotations).  A thread keeps a window of R entries in
-// registers and slides it by one per column ic; the block's rotations are
-// staged in LDS.
+// registers and slides it by one per column ic; the block's rotations (both
+// sides) are staged in LDS.  The rows of the four matrices, and the columns
+// of the two, are one task list per block: the rotation chains are latency
+// bound, so the pass is as fast as its most loaded wave and wants every wave.
+// Window entries in flight per row or column, and rotation coefficients per
+// compiler fence (16: one fence per sequence).  Measured on 2048 pencils of
+// order 32..512, far updates per workgroup: (4, 16) 93 ms, (8, 4) and (8, 16)
+// 100 ms; 262 ms with each entry loaded as it is needed.
+constexpr int kFarAhead = 4;
+constexpr int kCsAhead = 16;
+
+// The NQ rotation sequences of a block applied to one row or column of a
+// matrix: entry e of the window is p[e * stride], entries e < W exist, and
+// sequence icm (skipped where bit icm of `on` is clear) applies rotations
+// ig = R-1..1 to the pairs (icm+ig-1, icm+ig).  A window of R entries slides
+// by one per sequence; the entries it takes in are loaded D sequences ahead,
+// so the chain of rotations does not wait on memory.
+template <int R, int NQ, int D, bool FULL>
+__device__ __forceinline__ void apply_window_w(double* pg, size_t stride, int W, const double2* cs,
+                                               unsigned on) {
+  if (FULL) W = R + NQ - 1;  // every entry exists: no guard on any load or store
+  // The matrices are in global memory: global (not flat) loads and stores
+  // count only against vmcnt, so the waits for the LDS coefficients do not
+  // also drain the loads in flight.
+  __attribute__((address_space(1))) double* p = (__attribute__((address_space(1))) double*)pg;
+  typedef const __attribute__((address_space(3))) double lds_double;
+  lds_double* csl = (lds_double*)cs;  // (c, s) pairs
+  double v[R], pf[D];
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = k < W ? p[k * stride] : 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) pf[d] = R + d < W ? p[(R + d) * stride] : 0.0;
+#pragma unroll
+  for (int icm = 0; icm < NQ; ++icm) {
+    if ((on >> icm) & 1u) {
+#pragma unroll
+      for (int ig = R - 1; ig >= 1; --ig) {
+        // at most kCsAhead coefficients loaded ahead: the compiler would
+        // otherwise load far more (all NQ * R of them, ahead of the loop
+        // over tasks, where they are the same for every task) and spill
+        if ((R - 1 - ig) % kCsAhead == 0) asm volatile("" : "+v"(csl));
+        const double cc = csl[2 * (icm * R + ig)], sn = csl[2 * (icm * R + ig) + 1];
+        const double x = v[ig], y = v[ig - 1];
+        v[ig] = cc * x + sn * y;
+        v[ig - 1] = cc * y - sn * x;
+      }
+    }
+    if (icm < W) p[icm * stride] = v[0];
+#pragma unroll
+    for (int k = 0; k < R - 1; ++k) v[k] = v[k + 1];
+    v[R - 1] = pf[icm % D];  // entry R + icm
+    const int e = R + icm + D;
+    pf[icm % D] = e < W ? p[e * stride] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < R - 1; ++k)
+    if (NQ + k < W) p[(NQ + k) * stride] = v[k];
+}
+
+template <int R, int NQ, int D>
+__device__ __forceinline__ void apply_window(double* p, size_t stride, int W, const double2* cs,
+                                             unsigned on) {
+  if (W == R + NQ - 1)
+    apply_window_w<R, NQ, D, true>(p, stride, W, cs, on);
+  else
+    apply_window_w<R, NQ, D, false>(p, stride, W, cs, on);
+}
+
 template <int R, int NQ>
 __device__ __noinline__ void far_updates_wg(double* sm, double* A, double* B, double* Q, double* Z, int n,
                                int s, int nblocks, const double* CU, const double* SU,
                                const double* CV, const double* SV) {
   constexpr int LDR = R + NQ;
-  double2* cs = reinterpret_cast<double2*>(sm);  // [NQ][R]
+  double2* csv = reinterpret_cast<double2*>(sm);  // [NQ][R] right rotations (rows of A, B, Z)
+  double2* csu = csv + NQ * R;                     // [NQ][R] left rotations (rows of Q, columns)
   const int tid = threadIdx.x;
-  // rows: which 0 A, 1 B (partial), 2 Q, 3 Z; a pass covers kBT rows
-  for (int which = 0; which < 4; ++which) {
-    double* M = which == 0 ? A : which == 1 ? B : which == 2 ? Q : Z;
-    const double* Ct = which == 2 ? CU : CV;
-    const double* St = which == 2 ? SU : SV;
-    const double sgn = which == 2 ? -1.0 : 1.0;
-    for (int r0 = 0; r0 < n; r0 += kBT) {
-      const int i = r0 + tid;
-      const bool valid = i < n;
-      for (int ib = nblocks; ib >= 1; --ib) {
-        const int cw0 = s + (ib - 1) * R;  // 0-based first window column
-        const int W = min(R + NQ - 1, n - cw0);
-        if (W <= 0) continue;
-        if (which < 2 && r0 + 1 >= max(s + 1, s + 1 + (ib - 1) * R)) continue;
-        __syncthreads();  // the previous block's coefficients are consumed
-        for (int e = tid; e < NQ * R; e += kBT) {
-          const int icm = e / R, ig = e % R;
-          const size_t ti = (size_t)(ib - 1) * LDR * NQ + (size_t)icm * LDR + (icm + ig - 1);
-          cs[e] = (ig >= 1) ? make_double2(Ct[ti], sgn * St[ti]) : make_double2(1.0, 0.0);
-        }
-        __syncthreads();
-        double v[R];
+  // One task per row, over the rows of all four matrices at once, so that a
+  // block's pass keeps the whole workgroup busy: tasks [0, n) are Z's rows,
+  // [n, 2n) Q's, then the rows of A and of B above the block window (only
+  // those meet a rotation of the block).  Each row sees the blocks in the
+  // order ib = nblocks..1 as in the sequential update.
+  for (int ib = nblocks; ib >= 1; --ib) {
+    const int cw0 = s + (ib - 1) * R;  // 0-based first window column
+    const int W = min(R + NQ - 1, n - cw0);
+    if (W <= 0) continue;
+    const int na = min(n, s + (ib - 1) * R);  // rows of A (and of B) above the window
+    __syncthreads();  // the previous block's coefficients are consumed
+    for (int e = tid; e < NQ * R; e += kBT) {
+      const int icm = e / R, ig = e % R;
+      const size_t ti = (size_t)(ib - 1) * LDR * NQ + (size_t)icm * LDR + (icm + ig - 1);
+      csv[e] = (ig >= 1) ? make_double2(CV[ti], SV[ti]) : make_double2(1.0, 0.0);
+      csu[e] = (ig >= 1) ? make_double2(CU[ti], -SU[ti]) : make_double2(1.0, 0.0);
+    }
+    __syncthreads();
+    const int ntask = 2 * n + 2 * na;
+    for (int t = tid; t < ntask; t += kBT) {
+      const int which = t < n ? 3 : t < 2 * n ? 2 : t < 2 * n + na ? 0 : 1;  // 0 A, 1 B, 2 Q, 3 Z
+      const int i = t < 2 * n ? t - (which == 3 ? 0 : n) : t - 2 * n - (which == 0 ? 0 : na);
+      double* M = which == 0 ? A : which == 1 ? B : which == 2 ? Q : Z;
+      const double2* cs = which == 2 ? csu : csv;
+      unsigned on = 0;
 #pragma unroll
-        for (int k = 0; k < R; ++k) v[k] = (valid && k < W) ? M[PHT_IDX(i, cw0 + k, n)] : 0.0;
-        for (int icm = 0; icm < NQ; ++icm) {
-          bool act = valid;
-          if (which < 2) act = act && (i + 1 < max(s + 1, s + 1 + (ib - NQ + icm) * R));
-          if (act) {
-#pragma unroll
-            for (int ig = R - 1; ig >= 1; --ig) {
-              const double2 c = cs[icm * R + ig];
-              const double x = v[ig], y = v[ig - 1];
-              v[ig] = c.x * x + c.y * y;
-              v[ig - 1] = c.x * y - c.y * x;
-            }
-          }
-          if (valid && icm < W) M[PHT_IDX(i, cw0 + icm, n)] = v[0];
-#pragma unroll
-          for (int k = 0; k < R - 1; ++k) v[k] = v[k + 1];
-          v[R - 1] = (valid && R + icm < W) ? M[PHT_IDX(i, cw0 + R + icm, n)] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < R - 1; ++k)
-          if (valid && NQ + k < W) M[PHT_IDX(i, cw0 + NQ + k, n)] = v[k];
-      }
+      for (int icm = 0; icm < NQ; ++icm)
+        on |= (which >= 2 || i + 1 < max(s + 1, s + 1 + (ib - NQ + icm) * R)) ? (1u << icm) : 0u;
+      apply_window<R, NQ, kFarAhead>(&M[PHT_IDX(i, cw0, n)], (size_t)n, W, cs, on);
     }
   }
-  __threadfence_block();
-  // columns: which 0 A, 1 B; a pass covers kBT columns
-  for (int which = 0; which < 2; ++which) {
-    double* M = which == 0 ? A : B;
-    for (int c0 = 0; c0 < n; c0 += kBT) {
-      const int j = c0 + tid;
-      for (int ib = nblocks; ib >= 1; --ib) {
-        const int istop = (which == 0) ? ((ib == 1) ? s + NQ - 1 : s + (ib - 2) * R + NQ)
-                                       : s + (ib - 1) * R + R + NQ - 1;  // 1-based
-        if (c0 + kBT <= istop) continue;
-        const int rw0 = s + (ib - 1) * R;  // 0-based first row
-        const int W = min(R + NQ - 1, n - rw0);
-        if (W <= 0) continue;
-        __syncthreads();
-        for (int e = tid; e < NQ * R; e += kBT) {
-          const int icm = e / R, ig = e % R;
-          const size_t ti = (size_t)(ib - 1) * LDR * NQ + (size_t)icm * LDR + (icm + ig - 1);
-          cs[e] = (ig >= 1) ? make_double2(CU[ti], -SU[ti]) : make_double2(1.0, 0.0);
-        }
-        __syncthreads();
-        const bool act = (j < n) && (j + 1 > istop);
-        if (!act) continue;
-        double v[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) v[k] = (k < W) ? M[PHT_IDX(rw0 + k, j, n)] : 0.0;
-        for (int icm = 0; icm < NQ; ++icm) {
-#pragma unroll
-          for (int ig = R - 1; ig >= 1; --ig) {
-            const double2 c = cs[icm * R + ig];
-            const double x = v[ig], y = v[ig - 1];
-            v[ig] = c.x * x + c.y * y;
-            v[ig - 1] = c.x * y - c.y * x;
-          }
-          if (icm < W) M[PHT_IDX(rw0 + icm, j, n)] = v[0];
-#pragma unroll
-          for (int k = 0; k < R - 1; ++k) v[k] = v[k + 1];
-          v[R - 1] = (R + icm < W) ? M[PHT_IDX(rw0 + R + icm, j, n)] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < R - 1; ++k)
-          if (NQ + k < W) M[PHT_IDX(rw0 + NQ + k, j, n)] = v[k];
-      }
+  __syncthreads();
+  // Columns of A and of B right of each block's stop column, one task per
+  // column of both matrices, blocks in the order ib = nblocks..1.
+  for (int ib = nblocks; ib >= 1; --ib) {
+    const int rw0 = s + (ib - 1) * R;  // 0-based first row
+    const int W = min(R + NQ - 1, n - rw0);
+    if (W <= 0) continue;
+    const int istopa = (ib == 1) ? s + NQ - 1 : s + (ib - 2) * R + NQ;  // 1-based
+    const int istopb = s + (ib - 1) * R + R + NQ - 1;
+    const int nca = max(0, n - istopa), ncb = max(0, n - istopb);  // columns j >= istop (0-based)
+    if (nca + ncb == 0) continue;
+    __syncthreads();
+    for (int e = tid; e < NQ * R; e += kBT) {
+      const int icm = e / R, ig = e % R;
+      const size_t ti = (size_t)(ib - 1) * LDR * NQ + (size_t)icm * LDR + (icm + ig - 1);
+      csu[e] = (ig >= 1) ? make_double2(CU[ti], -SU[ti]) : make_double2(1.0, 0.0);
+    }
+    __syncthreads();
+    for (int t = tid; t < nca + ncb; t += kBT) {
+      double* M = t < nca ? A : B;
+      const int j = t < nca ? istopa + t : istopb + (t - nca);
+      apply_window<R, NQ, kFarAhead>(&M[PHT_IDX(rw0, j, n)], 1, W, csu, ~0u);
     }
   }
   __threadfence_block();

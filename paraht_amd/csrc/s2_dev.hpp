@@ -40,9 +40,16 @@ __device__ __forceinline__ void rot_from_norm(double f, double g, double d,
 // A window row with fewer than RMAX columns (the last block of a sweep): kept
 // out of line so that its per-column tests and addresses do not take
 // registers in the kernels that inline the full-block path.
+// Mx is in global memory.  Saying so makes its accesses global_* rather than
+// flat_* instructions: a flat access counts against the LDS counter too, so
+// every later wait for an LDS read would also wait for it to complete.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gdouble* as_global(double* p) { return (gdouble*)p; }
+
 template <int RMAX>
-__device__ __noinline__ void rotate_row_partial(double* __restrict__ Mx, int ld, int rr0, int R0,
+__device__ __noinline__ void rotate_row_partial(double* __restrict__ Mg, int ld, int rr0, int R0,
                                                 int nb, const double* rc, const double* rs) {
+  gdouble* Mx = as_global(Mg);
   double seg[RMAX];
 #pragma unroll
   for (int j = 0; j < RMAX; ++j) seg[j] = (j < nb) ? Mx[PHT_IDX(rr0, R0 + j, ld)] : 0.0;
@@ -76,7 +83,7 @@ __device__ __forceinline__ void rotate_row(double* __restrict__ Mx, int ld, int 
   // the row's RMAX entries by a pointer walk (no per-column tests or address
   // tables), all loads in flight together
   double seg[RMAX];
-  double* p = Mx + PHT_IDX(rr0, R0, ld);
+  gdouble* p = as_global(Mx + PHT_IDX(rr0, R0, ld));
 #pragma unroll
   for (int j = 0; j < RMAX; ++j) {
     seg[j] = *p;
