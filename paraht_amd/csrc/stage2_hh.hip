@@ -136,7 +136,7 @@ __device__ __noinline__ void hh_rows_above(const HhSweepArgs& a, int istart, int
   const int nr = row0 - istart;
   for (int e = first; e < 2 * nr; e += nthr) {
     const bool inA = e < nr;
-    double* M = inA ? a.A : a.B;
+    s2::gdouble* M = s2::as_global(inA ? a.A : a.B);
     const int ld = inA ? a.lda : a.ldb;
     const size_t base = (size_t)(istart - 1 + (inA ? e : e - nr)) + (size_t)(row0 - 1) * ld;
     double x[RMAX];
@@ -168,8 +168,10 @@ __device__ void hh_step(const HhSweepArgs& a, int ic, int ib, double* L, bool ro
   const int nb = imin(r, n - row0 + 1);
   const int bcol = row0 + r - 1;
   const bool gen = nb >= 2;
-  double* Ub = a.U + (size_t)(ib - 1) * ldu * nq;
-  double* Vb = a.V + (size_t)(ib - 1) * ldu * nq;
+  // tape and matrices through global-typed pointers: global_* rather than
+  // flat_* accesses, whose completion every LDS wait would also wait for
+  s2::gdouble* Ub = s2::as_global(a.U) + (size_t)(ib - 1) * ldu * nq;
+  s2::gdouble* Vb = s2::as_global(a.V) + (size_t)(ib - 1) * ldu * nq;
   double* Bl = L + Lay::kBl;
   double* Vr = L + Lay::kVr;
   double* vb = L + Lay::kVb;
@@ -191,14 +193,14 @@ __device__ void hh_step(const HhSweepArgs& a, int ic, int ib, double* L, bool ro
   // replaced by zeros afterwards, so every load is unconditional and all of
   // them are in flight together.
   for (int e = t; e < nlazy * ldu; e += kHT) Uc[e] = Ub[e];
-  if (t < nlazy) tauUc[t] = a.tauU[(size_t)(ib - 1) * nq + t];
+  if (t < nlazy) tauUc[t] = s2::as_global(a.tauU)[(size_t)(ib - 1) * nq + t];
   const int nrw = gen ? imin(n, row0 + r + nb - 1) - row0 + 1 : 0;  // rows of the A window
   if (gen) {
     // all loads of both tiles in flight together, then the LDS stores
     constexpr int KB = RMAX * RMAX / kHT, KA = 2 * RMAX * RMAX / kHT;
     double vb_[KB], va_[KA];
-    const double* Bb = a.B + (size_t)(row0 - 1) + (size_t)(row0 - 1) * a.ldb;
-    const double* Ab = a.A + (size_t)(row0 - 1) + (size_t)(row0 - 1) * a.lda;
+    const s2::gdouble* Bb = s2::as_global(a.B) + (size_t)(row0 - 1) + (size_t)(row0 - 1) * a.ldb;
+    const s2::gdouble* Ab = s2::as_global(a.A) + (size_t)(row0 - 1) + (size_t)(row0 - 1) * a.lda;
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
       const int e = t + k * kHT;
@@ -229,8 +231,8 @@ __device__ void hh_step(const HhSweepArgs& a, int ic, int ib, double* L, bool ro
   const int lb = (bcol <= n) ? ic + r - 2 : 0;
   double x0 = 0.0, x1 = 0.0;
   if ((wave == 0 && la > 0) || (wave == 1 && lb > 0)) {
-    const double* cp = wave == 0 ? a.A + (size_t)g0 + (size_t)(col - 1) * a.lda
-                                 : a.B + (size_t)g0 + (size_t)(bcol - 1) * a.ldb;
+    const s2::gdouble* cp = s2::as_global(wave == 0 ? a.A + (size_t)g0 + (size_t)(col - 1) * a.lda
+                                                    : a.B + (size_t)g0 + (size_t)(bcol - 1) * a.ldb);
     const int len = wave == 0 ? la : lb;
     const double v0 = cp[imin(lane, len - 1)], v1 = cp[imin(lane + 64, len - 1)];
     x0 = lane < len ? v0 : 0.0;
@@ -255,7 +257,7 @@ __device__ void hh_step(const HhSweepArgs& a, int ic, int ib, double* L, bool ro
       const double al = readlane_d(p < 64 ? x0 : x1, p & 63);
       const bool in0 = lane > p && lane < p + nb, in1 = lane + 64 > p && lane + 64 < p + nb;
       const Refl F = wave_refl(al, x0, in0, x1, in1);
-      double* ucol = Ub + (size_t)(ic - 1) * ldu;
+      s2::gdouble* ucol = Ub + (size_t)(ic - 1) * ldu;
       // u: 1 at index p, x * scal below; A's entries below p become zeros
       if (in0) {
         const double u = x0 * F.scal;
@@ -276,18 +278,18 @@ __device__ void hh_step(const HhSweepArgs& a, int ic, int ib, double* L, bool ro
           x1 = F.beta;
         ucol[p] = 1.0;
         uvec[0] = 1.0;
-        a.tauU[(size_t)(ib - 1) * nq + ic - 1] = F.tau;
+        s2::as_global(a.tauU)[(size_t)(ib - 1) * nq + ic - 1] = F.tau;
         sc[0] = F.tau;
       }
       if (lane >= nb && lane < RMAX) uvec[lane] = 0.0;
     }
-    double* cp = a.A + (size_t)g0 + (size_t)(col - 1) * a.lda;
+    s2::gdouble* cp = s2::as_global(a.A + (size_t)g0 + (size_t)(col - 1) * a.lda);
     if (lane < la) cp[lane] = x0;
     if (lane + 64 < la) cp[lane + 64] = x1;
   } else if (wave == 1 && lb > 0) {
     // B's column bcol: rows above the block to memory, block rows (the block's
     // last column, nb == r here) into the LDS block
-    double* cp = a.B + (size_t)g0 + (size_t)(bcol - 1) * a.ldb;
+    s2::gdouble* cp = s2::as_global(a.B + (size_t)g0 + (size_t)(bcol - 1) * a.ldb);
     const int above = ic - 1;
     if (lane < above) cp[lane] = x0;
     if (lane + 64 < above) cp[lane + 64] = x1;
@@ -376,12 +378,12 @@ __device__ void hh_step(const HhSweepArgs& a, int ic, int ib, double* L, bool ro
     const double al = readlane_d(x, 0);
     const bool in0 = lane >= 1 && lane < nb;
     const Refl F = wave_refl(al, x, in0, 0.0, false);
-    double* vcol = Vb + (size_t)(ic - 1) * ldu + (ic - 1);
+    s2::gdouble* vcol = Vb + (size_t)(ic - 1) * ldu + (ic - 1);
     const double v = (lane == 0) ? 1.0 : (in0 ? x * F.scal : 0.0);
     if (lane < RMAX) vvec[lane] = v;
     if (lane < nb) vcol[lane] = v;
     if (lane == 0) {
-      a.tauV[(size_t)(ib - 1) * nq + ic - 1] = F.tau;
+      s2::as_global(a.tauV)[(size_t)(ib - 1) * nq + ic - 1] = F.tau;
       sc[1] = F.tau;
     }
   }
@@ -427,14 +429,14 @@ __device__ void hh_step(const HhSweepArgs& a, int ic, int ib, double* L, bool ro
 
   // ---- stores
   {
-    double* Bb = a.B + (size_t)(row0 - 1) + (size_t)(row0 - 1) * a.ldb;
+    s2::gdouble* Bb = s2::as_global(a.B + (size_t)(row0 - 1) + (size_t)(row0 - 1) * a.ldb);
 #pragma unroll 4
     for (int k = 0; k < RMAX * RMAX / kHT; ++k) {
       const int e = t + k * kHT;
       const int m = e % RMAX, c = e / RMAX;
       if (m < nb && c < nb) Bb[(size_t)m + (size_t)c * a.ldb] = Bl[c * LD + m];
     }
-    double* Ab = a.A + (size_t)(row0 - 1) + (size_t)(row0 - 1) * a.lda;
+    s2::gdouble* Ab = s2::as_global(a.A + (size_t)(row0 - 1) + (size_t)(row0 - 1) * a.lda);
 #pragma unroll 4
     for (int k = 0; k < 2 * RMAX * RMAX / kHT; ++k) {
       const int e = t + k * kHT;
@@ -466,10 +468,11 @@ __device__ void hh_helper(const HhSweepArgs& a, int ic, int h, double* L) {
     const int nb = imin(r, n - row0 + 1);
     const int istart = (ib == 1) ? a.s + 1 : imax(a.s + 1, a.s + 1 + (ib - 1 - nq + ic) * r);
     if (nb >= 2 && istart < row0) {
-      const double* vcol = a.V + (size_t)(ib - 1) * a.ldu * nq + (size_t)(ic - 1) * a.ldu + (ic - 1);
+      const s2::gdouble* vcol =
+          s2::as_global(a.V + (size_t)(ib - 1) * a.ldu * nq + (size_t)(ic - 1) * a.ldu + (ic - 1));
       if (t < RMAX) hv[t] = t < nb ? vcol[t] : 0.0;
       __syncthreads();
-      const double tau = a.tauV[(size_t)(ib - 1) * nq + ic - 1];
+      const double tau = s2::as_global(a.tauV)[(size_t)(ib - 1) * nq + ic - 1];
       hh_rows_above<RMAX>(a, istart, row0, nb, hv, tau, h * kHT + t, a.nhelp * kHT);
     }
     publish(mine, ib);
