@@ -330,8 +330,9 @@ __device__ __noinline__ void stage2_wg(double* sm, double* A, double* B, double*
             __builtin_amdgcn_s_sleep(1);
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
-        chase_step<kBRmax, kGT>(grp, A, n, B, n, n, r, nq, s, ic, ib, col, row0, CU, SU, CV, SV,
-                                ldr, Mw, true, Bb, lc, ls, rc, rs, xa, xb, red, Ml);
+        chase_step<kBRmax, kGT, WaveGroup, 2 * kBRmax>(grp, A, n, B, n, n, r, nq, s, ic, ib, col,
+                                                       row0, CU, SU, CV, SV, ldr, Mw, true, Bb, lc,
+                                                       ls, rc, rs, xa, xb, red, Ml);
         grp.sync();  // the step's stores are done
         if (grp.tid() == 0)
           __hip_atomic_store(&prog[ic], ib, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);

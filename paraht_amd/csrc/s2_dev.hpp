@@ -172,20 +172,22 @@ __device__ __forceinline__ void left_chain_lds(double* p, int st, int nb, const 
 
 // M of one block (ldr x ldr, column-major, identity for the first column
 // ic = 1) into LDS (ld ldr+1) by nw waves starting at wave w0 (of the thread
-// group whose thread tid calls it): a wave per set
-// of columns, lanes along the rows (ldr <= kMaxLdr = 160 rows: three 64-row
-// chunks), 8 columns' loads in flight together.
+// group whose thread tid calls it): a wave per set of columns, lanes along
+// the rows (ldr <= LDRMAX rows: (LDRMAX+63)/64 chunks of 64, three at
+// kMaxLdr = 160), 8 columns' loads in flight together.
+template <int LDRMAX = kMaxLdr>
 __device__ __forceinline__ void stage_m(double* Ml, const double* __restrict__ M, int ldr,
                                         int ic, int w0, int nw, int tid) {
+  constexpr int CH = (LDRMAX + 63) / 64;
   const int lane = tid & 63, wave = (tid >> 6) - w0;
   const int ldl = ldr + 1;
   for (int cm0 = wave; cm0 < ldr; cm0 += 8 * nw) {
-    double tmp[8][3];
+    double tmp[8][CH];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int cm = cm0 + u * nw;
 #pragma unroll
-      for (int h = 0; h < 3; ++h) {
+      for (int h = 0; h < CH; ++h) {
         const int rw = lane + 64 * h;
         tmp[u][h] = (cm < ldr && rw < ldr)
                         ? ((ic > 1) ? M[(size_t)cm * ldr + rw] : (rw == cm ? 1.0 : 0.0))
@@ -196,7 +198,7 @@ __device__ __forceinline__ void stage_m(double* Ml, const double* __restrict__ M
     for (int u = 0; u < 8; ++u) {
       const int cm = cm0 + u * nw;
 #pragma unroll
-      for (int h = 0; h < 3; ++h) {
+      for (int h = 0; h < CH; ++h) {
         const int rw = lane + 64 * h;
         if (cm < ldr && rw < ldr) Ml[cm * ldl + rw] = tmp[u][h];
       }
@@ -252,7 +254,8 @@ struct WaveGroup {
 // buffer), waves 1.. stage M of block ib+1 into it while wave 0 runs the right
 // chain; the caller must have acquired that block's M (the previous column's
 // step ib+1 is complete).  Returns whether Mnext was filled.
-template <int RMAX, int NT, class Grp>
+// LDRMAX bounds ldr = r + nq (the M staging holds (LDRMAX+63)/64 chunks).
+template <int RMAX, int NT, class Grp, int LDRMAX = kMaxLdr>
 __device__ inline bool chase_step(Grp& grp, double* __restrict__ A, int lda, double* __restrict__ B,
                                   int ldb, int n, int r, int nq, int s, int ic, int ib, int col,
                                   int row0, double* __restrict__ CU, double* __restrict__ SU,
@@ -273,7 +276,7 @@ __device__ inline bool chase_step(Grp& grp, double* __restrict__ A, int lda, dou
   double* M = Mw + (size_t)(ib - 1) * msz;
 
   // ---- M of this block into LDS (unless the previous step staged it)
-  if (!have_m) stage_m(Ml, M, ldr, ic, 0, NT / 64, tid);
+  if (!have_m) stage_m<LDRMAX>(Ml, M, ldr, ic, 0, NT / 64, tid);
   grp.sync();
   bool staged_next = false;
 
@@ -392,7 +395,7 @@ __device__ inline bool chase_step(Grp& grp, double* __restrict__ A, int lda, dou
         const int k = e % RMAX, cm = e / RMAX;
         if (k < nb && cm < ldr) M[(ic - 1) + k + (size_t)cm * ldr] = Ml[cm * ldl + (ic - 1) + k];
       }
-      if (NT > 64 && Mnext != nullptr) stage_m(Mnext, M + msz, ldr, ic, 1, NT / 64 - 1, tid);
+      if (NT > 64 && Mnext != nullptr) stage_m<LDRMAX>(Mnext, M + msz, ldr, ic, 1, NT / 64 - 1, tid);
     }
     staged_next = NT > 64 && Mnext != nullptr;
 
