@@ -71,6 +71,20 @@ __device__ void apply_wg(double* C, int ldc, const double* V, int ldv, const dou
   __syncthreads();
 }
 
+// The panel QR and the RQ generator out of line: inlined into stage1_wg they
+// shared its register allocation and spilled (1.5k scratch reloads in the
+// block loop at 128 registers); on their own, like the single-pencil
+// generator kernels, they do not.
+__device__ __noinline__ void panel_qr_wg(double* sm, double* A, int n, int jb, int nb, int p,
+                                         int nblocks, double* VL, int P, double* TL) {
+  s1::panel_qr_body(sm, A, n, n, jb, nb, p, nblocks, VL, P, (long long)P * nb, TL, nb,
+                    (long long)nb * nb);
+}
+__device__ __noinline__ void rq_gen_wg(double* sm, const double* B, int n, int i0, int len, int nb,
+                                       double* V, int P, double* T) {
+  s1::rq_gen_body(sm, B, n, i0, len, nb, V, P, T, nb);
+}
+
 __device__ __noinline__ void stage1_wg(double* sm, double* A, double* B, double* Q, double* Z, int n, int nb,
                           int p, double* ws) {
   const int P = p * nb;
@@ -84,8 +98,7 @@ __device__ __noinline__ void stage1_wg(double* sm, double* A, double* B, double*
     if ((n - nb - jb) % ((p - 1) * nb)) ++nblocks;
     const PanelGeom g{n, nb, p, jb, nblocks, P};
     // left generation and application (ab2rht.f90:85-250)
-    s1::panel_qr_body(sm, A, n, n, jb, nb, p, nblocks, VL, P, (long long)P * nb, TL, nb,
-                      (long long)nb * nb);
+    panel_qr_wg(sm, A, n, jb, nb, p, nblocks, VL, P, TL);
     __syncthreads();
     apply_wg<kLeftT>(B, n, VL, P, TL, nb, g, 0, nblocks, -1, n, false, jb + nb, n);
     apply_wg<kLeftT>(A, n, VL, P, TL, nb, g, 0, nblocks, jb + nb, n, false, jb + nb, n);
@@ -93,7 +106,7 @@ __device__ __noinline__ void stage1_wg(double* sm, double* A, double* B, double*
     // right generation, block by block, with B's update (ab2rht.f90:270-336)
     for (int bl = 0; bl < nblocks; ++bl) {
       const ApplyBlock bk = g.block(bl, 0, -1, true);
-      s1::rq_gen_body(sm, B, n, bk.start, bk.len, nb, VR + bk.voff, P, TR + bk.toff, nb);
+      rq_gen_wg(sm, B, n, bk.start, bk.len, nb, VR + bk.voff, P, TR + bk.toff);
       __threadfence_block();
       __syncthreads();
       apply_wg<kRightN>(B, n, VR, P, TR, nb, g, bl, bl + 1, 0, -1, true, 0, bk.start + bk.len);
