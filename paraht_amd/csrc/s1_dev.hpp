@@ -251,7 +251,7 @@ __device__ __forceinline__ void gram_and_t(const double* Vs, int k, const double
 // With tau_out, each block's tau goes there (64 per block) and T is left to
 // panel_t_body, one workgroup per block after the chain; otherwise T is
 // built here, block by block.
-__device__ inline void panel_qr_body(
+__device__ __forceinline__ void panel_qr_body(
     double* sm, double* __restrict__ A, int lda, int n, int jb, int nb, int p, int nblocks,
     double* __restrict__ Vt, int ldv, long long vstride, double* __restrict__ Tt,
     int ldt, long long tstride, double* __restrict__ tau_out = nullptr) {
